@@ -1,0 +1,84 @@
+"""Device-array plumbing: torch tensors hold device memory and give the HIP
+stream; every computation on them goes through the C ABI."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._lib import NativeUnavailable
+
+DEVICE = "cuda"
+
+_NP2TORCH = {
+    np.dtype(np.float32): torch.float32,
+    np.dtype(np.float64): torch.float64,
+    np.dtype(np.int32): torch.int32,
+    np.dtype(np.complex64): torch.complex64,
+}
+_TORCH2NP = {v: k for k, v in _NP2TORCH.items()}
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise NativeUnavailable("no HIP device visible: the scico_amd kernels need an MI355X")
+
+
+def torch_dtype(dt) -> torch.dtype:
+    if isinstance(dt, torch.dtype):
+        return dt
+    return _NP2TORCH[np.dtype(dt)]
+
+
+def np_dtype(dt) -> np.dtype:
+    if isinstance(dt, torch.dtype):
+        return _TORCH2NP[dt]
+    return np.dtype(dt)
+
+
+def as_device(x, dtype=np.float32) -> torch.Tensor:
+    """Contiguous device tensor of ``dtype`` (copies host arrays to the GPU)."""
+    require_gpu()
+    td = torch_dtype(dtype)
+    if isinstance(x, torch.Tensor):
+        if x.device.type != "cuda":
+            x = x.to(DEVICE)
+        if x.dtype != td:
+            x = x.to(td)
+        return x.contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np_dtype(dtype)), device=DEVICE)
+
+
+def empty(shape, dtype=np.float32) -> torch.Tensor:
+    require_gpu()
+    return torch.empty(tuple(shape), dtype=torch_dtype(dtype), device=DEVICE)
+
+
+def zeros(shape, dtype=np.float32) -> torch.Tensor:
+    require_gpu()
+    return torch.zeros(tuple(shape), dtype=torch_dtype(dtype), device=DEVICE)
+
+
+def stream() -> int:
+    """Raw hipStream_t of torch's current stream (captured by graphs)."""
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else t.data_ptr()
+
+
+def to_numpy(x) -> np.ndarray:
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def array_dtype(x) -> np.dtype:
+    if isinstance(x, torch.Tensor):
+        return np_dtype(x.dtype)
+    return np.asarray(x).dtype
+
+
+def array_shape(x) -> tuple:
+    return tuple(x.shape)

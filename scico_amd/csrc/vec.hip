@@ -1,0 +1,205 @@
+// Elementwise vector kernels and deterministic float64 reductions used by
+// the generic LinearOperator algebra, functionals and solvers.
+//
+// Reductions: pass 1 writes one float64 partial per workgroup (fixed grid
+// of at most RED_BLOCKS workgroups, fixed traversal order), pass 2 sums the
+// partials in index order in a single workgroup.  No atomics, so results
+// are bitwise reproducible.
+#include "common.h"
+#include "vec_impl.h"
+
+using namespace scxr;
+
+namespace scxr {
+namespace vec {
+
+__global__ __launch_bounds__(256) void axpby_kernel(int64_t n, float a, const float* __restrict__ x,
+                                                    float b, const float* __restrict__ y,
+                                                    float* __restrict__ z, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    z[e] = __fadd_rn(__fmul_rn(a, x[e]), __fmul_rn(b, y[e]));
+}
+
+__global__ __launch_bounds__(256) void add_kernel(int64_t n, const float* __restrict__ x,
+                                                  float sign, const float* __restrict__ y,
+                                                  float* __restrict__ z, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    z[e] = sign > 0.0f ? __fadd_rn(x[e], y[e]) : __fsub_rn(x[e], y[e]);
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(int64_t n, float a, const float* __restrict__ x,
+                                                    float* __restrict__ z, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    z[e] = __fmul_rn(a, x[e]);
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(int64_t n, float v, float* __restrict__ z,
+                                                   const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) z[e] = v;
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(int64_t n, const float* __restrict__ x,
+                                                   float* __restrict__ z, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) z[e] = x[e];
+}
+
+// Element term of each reduction kind.
+__device__ __forceinline__ double red_term(int kind, const float* x, const float* y, int64_t e,
+                                           int64_t n, int K) {
+  switch (kind) {
+    case RED_DOT:
+      return static_cast<double>(x[e]) * static_cast<double>(y[e]);
+    case RED_SUMSQ: {
+      const double v = x[e];
+      return v * v;
+    }
+    case RED_SQDIST: {
+      const double v = static_cast<double>(x[e]) - static_cast<double>(y[e]);
+      return v * v;
+    }
+    case RED_SUM:
+      return x[e];
+    case RED_L1:
+      return fabs(static_cast<double>(x[e]));
+    case RED_L21: {  // x is (K, n): sum over columns of the l2 norm over axis 0
+      double s = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double v = x[static_cast<int64_t>(k) * n + e];
+        s += v * v;
+      }
+      return sqrt(s);
+    }
+    default:
+      return 0.0;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  // wave reduction then across the 4 waves, fixed order
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void reduce_pass1(int kind, int64_t n, int K,
+                                                    const float* __restrict__ x,
+                                                    const float* __restrict__ y,
+                                                    double* __restrict__ partial,
+                                                    const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    s += red_term(kind, x, y, e, n, K);
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void reduce_pass2(int nparts, const double* __restrict__ partial,
+                                                    double scale, int post,
+                                                    double* __restrict__ out,
+                                                    const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < nparts; e += 256) s += partial[e];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    s *= scale;
+    if (post == POST_SQRT) s = sqrt(s);
+    *out = s;
+  }
+}
+
+unsigned grid_for(int64_t n) {
+  const int64_t b = (n + 1023) / 1024;
+  return static_cast<unsigned>(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+int reduce(int kind, int64_t n, int K, const float* x, const float* y, double scale, int post,
+           double* out, double* ws, const int32_t* pred, hipStream_t s) {
+  const int64_t b = (n + 1023) / 1024;
+  const unsigned g = static_cast<unsigned>(b < 1 ? 1 : (b > RED_BLOCKS ? RED_BLOCKS : b));
+  hipLaunchKernelGGL(reduce_pass1, dim3(g), dim3(256), 0, s, kind, n, K, x, y, ws, pred);
+  SCXR_LAUNCHED("reduce pass 1");
+  hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(256), 0, s, static_cast<int>(g), ws, scale, post,
+                     out, pred);
+  SCXR_LAUNCHED("reduce pass 2");
+  return SCXR_OK;
+}
+
+}  // namespace vec
+}  // namespace scxr
+
+extern "C" {
+
+int scxr_vec_axpby(int64_t n, float a, const float* x, float b, const float* y, float* z,
+                   const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && y && z)), SCXR_ERR_ARG, "bad axpby arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::axpby_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     a, x, b, y, z, pred);
+  SCXR_LAUNCHED("axpby");
+  return SCXR_OK;
+}
+
+int scxr_vec_add(int64_t n, const float* x, float sign, const float* y, float* z,
+                 const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && y && z)), SCXR_ERR_ARG, "bad add arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::add_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     x, sign, y, z, pred);
+  SCXR_LAUNCHED("add");
+  return SCXR_OK;
+}
+
+int scxr_vec_scale(int64_t n, float a, const float* x, float* z, const int32_t* pred,
+                   void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && z)), SCXR_ERR_ARG, "bad scale arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::scale_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     a, x, z, pred);
+  SCXR_LAUNCHED("scale");
+  return SCXR_OK;
+}
+
+int scxr_vec_fill(int64_t n, float v, float* z, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || z), SCXR_ERR_ARG, "bad fill arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::fill_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     v, z, pred);
+  SCXR_LAUNCHED("fill");
+  return SCXR_OK;
+}
+
+int scxr_vec_copy(int64_t n, const float* x, float* z, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && z)), SCXR_ERR_ARG, "bad copy arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::copy_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     x, z, pred);
+  SCXR_LAUNCHED("copy");
+  return SCXR_OK;
+}
+
+int scxr_vec_reduce(int32_t kind, int64_t n, int32_t K, const float* x, const float* y,
+                    double scale, int32_t post, double* out, double* ws, const int32_t* pred,
+                    void* stream) {
+  SCXR_REQUIRE(kind >= vec::RED_DOT && kind <= vec::RED_L21, SCXR_ERR_ARG, "bad reduction kind");
+  SCXR_REQUIRE(n >= 0 && x && out && ws, SCXR_ERR_ARG, "bad reduction arguments");
+  SCXR_REQUIRE((kind != vec::RED_DOT && kind != vec::RED_SQDIST) || y, SCXR_ERR_ARG,
+               "reduction needs a second operand");
+  SCXR_REQUIRE(kind != vec::RED_L21 || K >= 1, SCXR_ERR_ARG, "L21 reduction needs K >= 1");
+  return vec::reduce(kind, n, K, x, y, scale, post, out, ws, pred, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// Internal interface of the 2D projector (shared by the 2D, per-slice 3D,
+// batched and solver translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace scxr {
+namespace xr2d {
+
+// Adjoint tiling: ADJ_TI x ADJ_TJ pixels per workgroup (one per thread),
+// ADJ_VC views staged per LDS chunk, at most ADJ_WMAX bins per window.
+constexpr int ADJ_TI = 16;
+constexpr int ADJ_TJ = 16;
+constexpr int ADJ_VC = 32;
+constexpr int ADJ_WMAX = 256;
+
+// Forward: 4 private LDS copies of (D + 4) bins per owned view.
+constexpr size_t FWD_LDS_MAX = 64 * 1024;
+inline size_t fwd_lds_bytes(int vpg, int D) {
+  return sizeof(float) * static_cast<size_t>(vpg) * 4 * (static_cast<size_t>(D) + 4);
+}
+
+struct Plan {
+  int M = 0, N = 0, A = 0, D = 0;
+  int vpg = 1;         // views per forward workgroup
+  int n_groups = 0;    // forward workgroups
+  int adj_w = 0;       // adjoint window width (bins)
+  bool any_colfast = false;
+  void* dev_block = nullptr;  // device tables (one allocation)
+  float4* geom = nullptr;     // (Pxmin, P0, P1, width) per view
+  double* rw = nullptr;       // RN64(1 / width) per view
+  int4* groups = nullptr;     // forward groups: (first, count, class, 0)
+  int* order = nullptr;       // views grouped by class
+  float* work = nullptr;      // transposed image workspace (M * N)
+  std::vector<float4> host_geom;
+};
+
+void view_geometry(int A, const float* angles, const double* x0, const double* dx, double y0,
+                   std::vector<float4>& geom, std::vector<double>& rw);
+int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx, double y0,
+           const float* angles);
+void destroy(Plan& p);
+int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s);
+int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s);
+
+}  // namespace xr2d
+}  // namespace scxr

@@ -1,0 +1,127 @@
+"""GPU parity of the 2D projector against the oracle (through the C ABI).
+
+Bar: detector bins and weights bit-exact; projections and back
+projections within rel-L2 1e-5 of the float32 oracle (the error against
+the float64 oracle is printed beside it); dot test <= 1e-5; bitwise
+determinism.  Full-size cases compare a subset of views (views are
+independent) plus size-independent properties.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import xray as X
+from scico_amd import phantom
+from scico_amd.linop import valid_adjoint
+from scico_amd.linop.xray import XRayTransform2D
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (shape, n_angles, kwargs)
+    ((12, 13), 10, {}),
+    ((16, 16), 3, {"det_count": int(16 * 1.05 / np.sqrt(2.0)), "dx": 1 / np.sqrt(2)}),
+    ((9, 7), 17, {"dx": 0.5}),
+    ((33, 65), 31, {"det_count": 20}),  # many bins out of range
+    ((64, 48), 45, {"det_count": 90, "y0": -40.0}),
+    ((256, 256), 180, {"det_count": 363}),  # 256^2, 180 views, 363 bins
+]
+
+
+def _op(shape, nang, kw):
+    ang = np.linspace(0, np.pi, nang, endpoint=False)
+    H = XRayTransform2D(shape, ang, **kw)
+    g = X.xr2d_defaults(shape, ang, x0=kw.get("x0"), dx=kw.get("dx"), y0=kw.get("y0"),
+                        det_count=kw.get("det_count"))
+    return H, g
+
+
+@pytest.mark.parametrize("shape,nang,kw", CASES)
+def test_view_table_bit_exact(gpu, shape, nang, kw):
+    H, g = _op(shape, nang, kw)
+    vt = H.view_table()
+    ref = np.stack(X.view_params(g["x0"], g["dx"], g["angles"], g["y0"]), axis=1)
+    np.testing.assert_array_equal(vt, ref)
+
+
+@pytest.mark.parametrize("shape,nang,kw", CASES)
+def test_weights_bit_exact(gpu, shape, nang, kw):
+    print(f"weights {shape} A={nang} {kw}", flush=True)
+    H, g = _op(shape, nang, kw)
+    inds, w = H.weights()
+    ri, rw = X.calc_weights(g["x0"], g["dx"], g["nx"], g["angles"], g["y0"])
+    np.testing.assert_array_equal(inds.cpu().numpy(), ri)
+    np.testing.assert_array_equal(w.cpu().numpy(), rw)
+
+
+@pytest.mark.parametrize("shape,nang,kw", CASES)
+def test_forward_adjoint_parity(gpu, shape, nang, kw):
+    print(f"fwd/adj {shape} A={nang} {kw}", flush=True)
+    H, g = _op(shape, nang, kw)
+    rng = np.random.default_rng(7)
+    x = rng.random(shape).astype(np.float32)
+    y = rng.standard_normal(H.output_shape).astype(np.float32)
+    ys = H(x).cpu().numpy()
+    bp = H.T(y).cpu().numpy()
+    ref_y = X.project(x, g["x0"], g["dx"], g["y0"], g["ny"], g["angles"])
+    ref_bp = X.back_project(y, g["x0"], g["dx"], g["nx"], g["y0"], g["angles"])
+    e_y, e_bp = rel_l2(ys, ref_y), rel_l2(bp, ref_bp)
+    y64 = X.project_f64(x, g["x0"], g["dx"], g["y0"], g["ny"], g["angles"])
+    bp64 = X.back_project_f64(y, g["x0"], g["dx"], g["nx"], g["y0"], g["angles"])
+    print(f"  rel-L2 vs fp32 oracle: fwd {e_y:.2e} adj {e_bp:.2e}; "
+          f"vs fp64: fwd {rel_l2(ys, y64):.2e} adj {rel_l2(bp, bp64):.2e}", flush=True)
+    assert e_y <= 1e-5 and e_bp <= 1e-5
+
+
+@pytest.mark.parametrize("shape,nang,kw", CASES[:3] + CASES[-1:])
+def test_dot_test(gpu, shape, nang, kw):
+    H, _ = _op(shape, nang, kw)
+    assert valid_adjoint(H, H.T, eps=1e-5)
+
+
+def test_determinism(gpu):
+    H = XRayTransform2D((256, 256), phantom.angles(180), det_count=363)
+    x = torch.as_tensor(phantom.ellipse_phantom(256, 10, seed=0), device="cuda")
+    y1, y2 = H(x), H(x)
+    b1, b2 = H.T(y1), H.T(y1)
+    assert torch.equal(y1, y2) and torch.equal(b1, b2)
+
+
+def test_full_size_512_720(gpu):
+    """512^2, 720 views, 724 bins: view-subset parity + mass + dot test."""
+    N, A, D = 512, 720, 724
+    ang = phantom.angles(A)
+    H = XRayTransform2D((N, N), ang, det_count=D)
+    x = phantom.ellipse_phantom(N, 20, seed=0)
+    xd = torch.as_tensor(x, device="cuda")
+    y = H(xd)
+    sub = np.arange(0, A, 37)
+    g = X.xr2d_defaults((N, N), ang[sub], det_count=D)
+    ref = X.project(x, g["x0"], g["dx"], g["y0"], g["ny"], g["angles"])
+    e = rel_l2(y.cpu().numpy()[sub], ref)
+    print(f"  512^2/720: view-subset rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-5
+    # no pixel-area factor and every pixel in range: each view sums to sum(x)
+    sums = y.double().sum(dim=1).cpu().numpy()
+    np.testing.assert_allclose(sums, np.full(A, x.astype(np.float64).sum()), rtol=1e-5)
+    # adjoint of a sinogram supported on the subset = oracle adjoint of the subset
+    ys = np.zeros((A, D), np.float32)
+    ys[sub] = np.random.default_rng(3).standard_normal((len(sub), D))
+    bp = H.T(ys).cpu().numpy()
+    ref_bp = X.back_project(ys[sub], g["x0"], g["dx"], g["nx"], g["y0"], g["angles"])
+    e = rel_l2(bp, ref_bp)
+    print(f"  512^2/720: adjoint subset rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-5
+    assert valid_adjoint(H, H.T, eps=1e-5)
+
+
+def test_shape_errors(gpu):
+    H = XRayTransform2D((8, 8), phantom.angles(4))
+    with pytest.raises(ValueError):
+        H(np.zeros((8, 9), np.float32))
+    with pytest.raises(ValueError):
+        H.adj(np.zeros((4, 5), np.float32))
+    with pytest.raises(ValueError):
+        H.adj(np.zeros(H.output_shape, np.float64))
