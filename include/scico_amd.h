@@ -89,6 +89,32 @@ int scxr_xr2d_adjoint(const scxr_xr2d_plan* plan, const float* sino, float* im,
 int scxr_xr2d_weights(const scxr_xr2d_plan* plan, int32_t* inds, float* weights,
                       void* stream);
 
+/* Launch facts: out[0] forward workgroups (= partial count of
+ * scxr_xr2d_forward_res), out[1] adjoint workgroups (= partial count of the
+ * adjoint dot epilogue), out[2] views per forward workgroup, out[3] adjoint
+ * detector window. */
+int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out /* host, 4 */);
+
+/* Forward projection that also writes, per forward workgroup, the partial
+ * sum of (y - A im)^2 over its sinogram rows (objective of
+ * SquaredL2Loss, scico/loss.py:208-209). */
+int scxr_xr2d_forward_res(const scxr_xr2d_plan* plan, const float* im, float* sino,
+                          const float* y, double* res_ws, const int32_t* pred, void* stream);
+
+/* Adjoint with a fused epilogue on the 2D finite-difference operator C
+ * (scico/linop/_diff.py, append=0 or circular):
+ *   kind 0: im = A^T sino
+ *   kind 1: im = rho * C^T C p + A^T sino     (ADMM x-step operator,
+ *           scico/optimize/_admmaux.py:222-229)
+ *   kind 2: im = A^T sino + rho * C^T (z - u) (ADMM x-step right-hand side,
+ *           scico/optimize/_admmaux.py:245-256; u may be NULL)
+ * If dot_ws is non-NULL, each adjoint workgroup writes its partial of
+ * sum(p * im) there (CG curvature p^T A p). */
+int scxr_xr2d_adjoint_epi(const scxr_xr2d_plan* plan, const float* sino, float* im,
+                          int32_t kind, float rho, int32_t circular, const float* p,
+                          const float* z, const float* u, double* dot_ws, const int32_t* pred,
+                          void* stream);
+
 
 /* ------------------------------------------------------ vector kernels --
  * Elementwise algebra behind LinearOperator sums / scalings
@@ -121,6 +147,65 @@ int scxr_vec_copy(int64_t n, const float* x, float* z, const int32_t* pred, void
 int scxr_vec_reduce(int32_t kind, int64_t n, int32_t K, const float* x, const float* y,
                     double scale, int32_t post, double* out, double* ws, const int32_t* pred,
                     void* stream);
+
+/* ---------------------------------------------- finite differences, prox --
+ * FiniteDifference (scico/linop/_diff.py:25-272) over all axes of an
+ * ndim-D array (ndim <= 3), append=0 (circular = 0) or circular; output
+ * stacked (ndim, *shape) as VerticalStack collapses it
+ * (scico/operator/_stack.py:126-128).
+ */
+int scxr_fd_forward(int32_t ndim, const int32_t* shape /* host */, const float* x, float* out,
+                    int32_t circular, const int32_t* pred, void* stream);
+/* out = scale * C^T g   (scico/linop/_stack.py:71-72) */
+int scxr_fd_adjoint(int32_t ndim, const int32_t* shape, const float* g, float* out,
+                    int32_t circular, float scale, const int32_t* pred, void* stream);
+/* out = rho * C^T C x   (rho * FiniteDifference.gram_op) */
+int scxr_fd_gram(int32_t ndim, const int32_t* shape, float rho, const float* x, float* out,
+                 int32_t circular, const int32_t* pred, void* stream);
+/* L21Norm.prox over axis 0 of v (K, n), K <= 8 (scico/functional/_norm.py:254-263) */
+int scxr_l21_prox(int32_t K, int64_t n, const float* v, float lam, float* out,
+                  const int32_t* pred, void* stream);
+/* L1Norm.prox (scico/functional/_norm.py:110-116) */
+int scxr_l1_prox(int64_t n, const float* v, float lam, float* out, const int32_t* pred,
+                 void* stream);
+
+/* ----------------------------------------------------------------- CG --
+ * scico.solver.cg (scico/solver.py:326-405) with device-resident scalars.
+ * state: float64[8 + 2 * (maxiter + 1)], flags: int32[maxiter + 1],
+ * ws: float64[2 * scxr_elem_blocks(n)].  Usage per solve:
+ *   Ax = A x0;  scxr_cg_start(b, Ax, r, p)
+ *   for k in [0, maxiter): Ap = A p (predicated on &flags[k]), with the
+ *     partials of p^T A p in pap_parts[pap_n];  scxr_cg_iter(k, ...)
+ * state[2] holds the iteration count, state[3] sqrt(r^T r) / ||b||.
+ */
+int scxr_cg_start(int64_t n, const float* b, const float* Ax, float* r, float* p, double* state,
+                  int32_t* flags, int32_t maxiter, double tol, double atol, double* ws,
+                  void* stream);
+int scxr_cg_iter(int64_t n, int32_t k, int32_t maxiter, float* x, float* r, float* p,
+                 const float* Ap, double* state, int32_t* flags, const double* pap_parts,
+                 int32_t pap_n, double* ws, void* stream);
+/* Workgroups (= partial count) of the elementwise passes over n elements. */
+int32_t scxr_elem_blocks(int64_t n);
+
+/* --------------------------------------------------------------- ADMM --
+ * Fused z/u update of scico.optimize.ADMM.step (scico/optimize/_admm.py:
+ * 364-378) for C = 2D FiniteDifference and g = lam * L21Norm (isotropic=1)
+ * or lam * L1Norm (isotropic=0); thr = lam / rho.  z, u, zold are (2, M, N);
+ * zold receives the previous z.  Partials (scxr_elem_blocks(M*N) each) of
+ * ||C x - z||^2 and g(z) / lam go to ws_prml and ws_g.
+ */
+int scxr_admm_zu2d(int32_t M, int32_t N, const float* x, float* z, float* u, float* zold,
+                   float thr, int32_t circular, int32_t isotropic, double* ws_prml,
+                   double* ws_g, void* stream);
+/* Partials of ||rho * C^T (z - zold)||^2 (dual residual, _admm.py:279-295). */
+int scxr_admm_dual2d(int32_t M, int32_t N, const float* z, const float* zold, float rho,
+                     int32_t circular, double* ws, void* stream);
+/* Appends (Objective, Prml Rsdl, Dual Rsdl, CG It, CG Res) to hist[5 * *counter]
+ * (scico/optimize/_admm.py:171-198, 241-295) and increments *counter. */
+int scxr_admm_itstat(const double* res, int32_t nres, const double* g, int32_t ng,
+                     const double* prml, int32_t nprml, const double* dual, int32_t ndual,
+                     const double* cg_state, double lam, double rho, double* hist,
+                     int32_t* counter, int32_t hist_cap, void* stream);
 
 #ifdef __cplusplus
 }

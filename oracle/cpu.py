@@ -1,0 +1,68 @@
+"""Python side of the C restatement (TEST INFRASTRUCTURE ONLY).
+
+Projector calls into ``libxray_cpu.so`` (OpenMP) and an ADMM-TV iteration
+built on them plus the NumPy operators of ``oracle.solvers``; timed by
+bench.py as the CPU baseline.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _cbuild
+from . import solvers as S
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _cbuild.load()
+    return _lib
+
+
+def _geom_args(g):
+    x0 = np.ascontiguousarray(np.asarray(g["x0"], dtype=np.float64))
+    dx = np.ascontiguousarray(np.asarray(g["dx"], dtype=np.float64))
+    ang = np.ascontiguousarray(np.asarray(g["angles"], dtype=np.float32))
+    return x0, dx, ang
+
+
+def weights(g):
+    M, N = g["nx"]
+    A = len(g["angles"])
+    x0, dx, ang = _geom_args(g)
+    inds = np.empty((A, M, N), np.int32)
+    w = np.empty((A, M, N), np.float32)
+    lib().cpu_xr2d_weights(M, N, A, x0.ctypes.data, dx.ctypes.data, float(g["y0"]),
+                           ang.ctypes.data, inds.ctypes.data, w.ctypes.data)
+    return inds, w
+
+
+def project(im, g):
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    M, N = im.shape
+    A, D = len(g["angles"]), g["ny"]
+    x0, dx, ang = _geom_args(g)
+    y = np.empty((A, D), np.float32)
+    lib().cpu_xr2d_forward(M, N, A, D, x0.ctypes.data, dx.ctypes.data, float(g["y0"]),
+                           ang.ctypes.data, im.ctypes.data, y.ctypes.data)
+    return y
+
+
+def back_project(y, g):
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    M, N = g["nx"]
+    A, D = y.shape
+    x0, dx, ang = _geom_args(g)
+    out = np.empty((M, N), np.float32)
+    lib().cpu_xr2d_adjoint(M, N, A, D, x0.ctypes.data, dx.ctypes.data, float(g["y0"]),
+                           ang.ctypes.data, y.ctypes.data, out.ctypes.data)
+    return out
+
+
+def admm_tv(y, g, lam, rho, x0, maxiter, cg_maxiter=10, cg_tol=1e-4):
+    """ADMM-TV (oracle.solvers.admm_tv) with the C/OpenMP projector."""
+    return S.admm_tv(y, lambda v: project(v, g), lambda v: back_project(v, g), lam, rho, x0,
+                     maxiter, cg_tol=cg_tol, cg_maxiter=cg_maxiter)

@@ -24,6 +24,7 @@ SOURCES = [
     "common.cpp",
     "xray2d.hip",
     "vec.hip",
+    "solver.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

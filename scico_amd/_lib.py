@@ -56,6 +56,38 @@ SIGNATURES = [
     ("scxr_xr2d_forward", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_adjoint", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_weights", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_xr2d_info", c_int32, [c_void_p, c_int32_p]),
+    ("scxr_xr2d_forward_res", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_xr2d_adjoint_epi", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_int32, c_float, c_int32, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p]),
+    # finite differences, proxes
+    ("scxr_fd_forward", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_fd_adjoint", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_int32, c_float, c_void_p, c_void_p]),
+    ("scxr_fd_gram", c_int32,
+     [c_int32, c_int32_p, c_float, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_l21_prox", c_int32, [c_int32, c_int64, c_void_p, c_float, c_void_p, c_void_p, c_void_p]),
+    ("scxr_l1_prox", c_int32, [c_int64, c_void_p, c_float, c_void_p, c_void_p, c_void_p]),
+    # CG
+    ("scxr_cg_start", c_int32,
+     [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double,
+      c_double, c_void_p, c_void_p]),
+    ("scxr_cg_iter", c_int32,
+     [c_int64, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_elem_blocks", c_int32, [c_int64]),
+    # ADMM
+    ("scxr_admm_zu2d", c_int32,
+     [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32,
+      c_void_p, c_void_p, c_void_p]),
+    ("scxr_admm_dual2d", c_int32,
+     [c_int32, c_int32, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
+    ("scxr_admm_itstat", c_int32,
+     [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
+      c_double, c_double, c_void_p, c_void_p, c_int32, c_void_p]),
     # vector kernels
     ("scxr_vec_axpby", c_int32,
      [c_int64, c_float, c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p]),

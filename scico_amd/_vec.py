@@ -9,6 +9,7 @@ from __future__ import annotations
 import torch
 
 from . import _device as dev
+from . import _pred
 from ._lib import call
 
 RED_DOT, RED_SUMSQ, RED_SQDIST, RED_SUM, RED_L1, RED_L21 = range(6)
@@ -24,32 +25,32 @@ def add(x, sign, y, out=None, pred=None):
     """``x + y`` (sign > 0) or ``x - y``."""
     out = torch.empty_like(x) if out is None else out
     call("scxr_vec_add", _n(x), x.data_ptr(), float(sign), y.data_ptr(), out.data_ptr(),
-         dev.ptr(pred), dev.stream())
+         dev.ptr(_pred.resolve(pred)), dev.stream())
     return out
 
 
 def axpby(a, x, b, y, out=None, pred=None):
     out = torch.empty_like(x) if out is None else out
     call("scxr_vec_axpby", _n(x), float(a), x.data_ptr(), float(b), y.data_ptr(), out.data_ptr(),
-         dev.ptr(pred), dev.stream())
+         dev.ptr(_pred.resolve(pred)), dev.stream())
     return out
 
 
 def scale(a, x, out=None, pred=None):
     out = torch.empty_like(x) if out is None else out
-    call("scxr_vec_scale", _n(x), float(a), x.data_ptr(), out.data_ptr(), dev.ptr(pred),
+    call("scxr_vec_scale", _n(x), float(a), x.data_ptr(), out.data_ptr(), dev.ptr(_pred.resolve(pred)),
          dev.stream())
     return out
 
 
 def fill(out, v, pred=None):
-    call("scxr_vec_fill", _n(out), float(v), out.data_ptr(), dev.ptr(pred), dev.stream())
+    call("scxr_vec_fill", _n(out), float(v), out.data_ptr(), dev.ptr(_pred.resolve(pred)), dev.stream())
     return out
 
 
 def copy(x, out=None, pred=None):
     out = torch.empty_like(x) if out is None else out
-    call("scxr_vec_copy", _n(x), x.data_ptr(), out.data_ptr(), dev.ptr(pred), dev.stream())
+    call("scxr_vec_copy", _n(x), x.data_ptr(), out.data_ptr(), dev.ptr(_pred.resolve(pred)), dev.stream())
     return out
 
 
@@ -62,7 +63,7 @@ def reduce(kind, x, y=None, K=1, scale_=1.0, post=POST_NONE, out=None, pred=None
     out = torch.empty((), dtype=torch.float64, device=dev.DEVICE) if out is None else out
     n = _n(x) // K if kind == RED_L21 else _n(x)
     call("scxr_vec_reduce", kind, n, K, x.data_ptr(), dev.ptr(y), float(scale_), post,
-         out.data_ptr(), workspace().data_ptr(), dev.ptr(pred), dev.stream())
+         out.data_ptr(), workspace().data_ptr(), dev.ptr(_pred.resolve(pred)), dev.stream())
     return out
 
 

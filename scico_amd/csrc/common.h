@@ -62,4 +62,34 @@ __device__ __forceinline__ float div_rn(float d, double rw) {
   return static_cast<float>(static_cast<double>(d) * rw);
 }
 
+// Sum of one double per thread over a 256-thread workgroup (fixed order:
+// wave shuffles, then the 4 wave sums left to right).  Thread 0 gets the
+// result.  `sh` holds 4 doubles of LDS.
+__device__ __forceinline__ double block_sum_f64(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  return r;
+}
+
+// Every thread of a 256-thread workgroup gets the same fixed-order sum of
+// parts[0..n).  Used to finish a reduction redundantly in each consumer
+// workgroup instead of in a separate launch.
+__device__ __forceinline__ double all_sum_f64(const double* __restrict__ parts, int n,
+                                              double* sh) {
+  double v = 0.0;
+  for (int e = threadIdx.x; e < n; e += 256) v += parts[e];
+  v = block_sum_f64(v, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) sh[0] = v;
+  __syncthreads();
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
 }  // namespace scxr

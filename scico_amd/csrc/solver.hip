@@ -1,0 +1,490 @@
+// Solver kernels: finite differences, L1 / L21 proximal maps, conjugate
+// gradient with device-resident scalars, and the fused ADMM z/u update
+// with its iteration statistics.
+//
+// CG (scico/solver.py:326-405) keeps every scalar on the device in a
+// float64 state block, so a fixed-length sequence of iterations can be
+// enqueued (or captured in a graph) without host synchronisation:
+//   state[ST_BN]      ||b||
+//   state[ST_TOL2]    max(tol * ||b||, atol)^2
+//   state[ST_NITER]   iterations performed
+//   state[ST_RELRES]  sqrt(num) / ||b||
+//   state[ST_NUM + k] r^T r after k iterations
+//   state[ST_NUM + (maxiter + 1) + k] p^T A p of iteration k
+//   flags[k]          1 while iteration k is to run ("ii < maxiter and
+//                     num > tol^2"), else 0; every kernel of iteration k is
+//                     predicated on flags[k].
+#include <cmath>
+
+#include "common.h"
+#include "fd_impl.h"
+#include "solver_impl.h"
+#include "vec_impl.h"
+
+using namespace scxr;
+
+namespace scxr {
+namespace solver {
+
+// ------------------------------------------------------ finite differences --
+
+// out (ndim, d0, d1, d2) = stack of per-axis differences of x (d0, d1, d2);
+// component c differences axis (3 - ndim) + c.
+__global__ __launch_bounds__(256) void fd_fwd_kernel(int ndim, int d0, int d1, int d2,
+                                                     const float* __restrict__ x,
+                                                     float* __restrict__ out, int circ,
+                                                     const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const int64_t n = static_cast<int64_t>(d0) * d1 * d2;
+  const int dims[3] = {d0, d1, d2};
+  const int64_t strides[3] = {static_cast<int64_t>(d1) * d2, d2, 1};
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int k0 = static_cast<int>(e / strides[0]);
+    const int k1 = static_cast<int>((e / d2) % d1);
+    const int k2 = static_cast<int>(e % d2);
+    const int idx[3] = {k0, k1, k2};
+    const float xc = x[e];
+    for (int c = 0; c < ndim; ++c) {
+      const int ax = 3 - ndim + c;
+      const int k = idx[ax], len = dims[ax];
+      const int kn = (k + 1 < len) ? k + 1 : 0;
+      const float xn = x[e + static_cast<int64_t>(kn - k) * strides[ax]];
+      out[c * n + e] = fd::diff_fwd(xc, xn, k, len, circ != 0);
+    }
+  }
+}
+
+// out = sum_c D_c^T g[c]   (scico/linop/_stack.py:71-72)
+__global__ __launch_bounds__(256) void fd_adj_kernel(int ndim, int d0, int d1, int d2,
+                                                     const float* __restrict__ g,
+                                                     float* __restrict__ out, int circ,
+                                                     float scale, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const int64_t n = static_cast<int64_t>(d0) * d1 * d2;
+  const int dims[3] = {d0, d1, d2};
+  const int64_t strides[3] = {static_cast<int64_t>(d1) * d2, d2, 1};
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int idx[3] = {static_cast<int>(e / strides[0]), static_cast<int>((e / d2) % d1),
+                        static_cast<int>(e % d2)};
+    float acc = 0.0f;
+    for (int c = 0; c < ndim; ++c) {
+      const int ax = 3 - ndim + c;
+      const int k = idx[ax], len = dims[ax];
+      const int km = (k >= 1) ? k - 1 : len - 1;
+      const float* gc = g + c * n;
+      const float t = fd::diff_adj(gc[e + static_cast<int64_t>(km - k) * strides[ax]], gc[e], k,
+                                   len, circ != 0);
+      acc = (c == 0) ? t : __fadd_rn(acc, t);
+    }
+    out[e] = scale == 1.0f ? acc : __fmul_rn(scale, acc);
+  }
+}
+
+// out = rho * C^T C x  (gram of the stacked differences)
+__global__ __launch_bounds__(256) void fd_gram_kernel(int ndim, int d0, int d1, int d2,
+                                                      const float* __restrict__ x,
+                                                      float* __restrict__ out, int circ,
+                                                      float rho, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const int64_t n = static_cast<int64_t>(d0) * d1 * d2;
+  const int dims[3] = {d0, d1, d2};
+  const int64_t strides[3] = {static_cast<int64_t>(d1) * d2, d2, 1};
+  const bool cc = circ != 0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int idx[3] = {static_cast<int>(e / strides[0]), static_cast<int>((e / d2) % d1),
+                        static_cast<int>(e % d2)};
+    const float xc = x[e];
+    float acc = 0.0f;
+    for (int c = 0; c < ndim; ++c) {
+      const int ax = 3 - ndim + c;
+      const int k = idx[ax], len = dims[ax];
+      const int kp = (k + 1 < len) ? k + 1 : 0;
+      const int km = (k >= 1) ? k - 1 : len - 1;
+      const int64_t st = strides[ax];
+      const float g0 = fd::diff_fwd(xc, x[e + (kp - k) * st], k, len, cc);
+      const float gm = fd::diff_fwd(x[e + (km - k) * st], xc, km, len, cc);
+      const float t = fd::diff_adj(gm, g0, k, len, cc);
+      acc = (c == 0) ? t : __fadd_rn(acc, t);
+    }
+    out[e] = __fmul_rn(rho, acc);
+  }
+}
+
+// -------------------------------------------------------------- proxes --
+
+// L21 over axis 0 of v (K, n)   (scico/functional/_norm.py:254-263)
+__device__ __forceinline__ void l21_shrink(float* v, int K, float lam) {
+  float ss = 0.0f;
+  for (int k = 0; k < K; ++k) ss = (k == 0) ? __fmul_rn(v[0], v[0]) : __fadd_rn(ss, __fmul_rn(v[k], v[k]));
+  const float len = sqrtf(ss);
+  float nl = __fsub_rn(len, lam);
+  nl = __fmul_rn(0.5f, __fadd_rn(nl, fabsf(nl)));
+  for (int k = 0; k < K; ++k) {
+    const float dir = (len != 0.0f) ? v[k] / len : 0.0f;  // no_nan_divide
+    v[k] = __fmul_rn(nl, dir);
+  }
+}
+
+// L1   (scico/functional/_norm.py:110-116)
+__device__ __forceinline__ float l1_shrink(float v, float lam) {
+  float t = __fsub_rn(fabsf(v), lam);
+  t = __fmul_rn(0.5f, __fadd_rn(t, fabsf(t)));
+  const float sg = (v > 0.0f) ? 1.0f : ((v < 0.0f) ? -1.0f : 0.0f);
+  return __fmul_rn(sg, t);
+}
+
+__global__ __launch_bounds__(256) void l21_prox_kernel(int K, int64_t n, const float* __restrict__ v,
+                                                       float lam, float* __restrict__ out,
+                                                       const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    float w[MAX_K];
+    for (int k = 0; k < K; ++k) w[k] = v[k * n + e];
+    l21_shrink(w, K, lam);
+    for (int k = 0; k < K; ++k) out[k * n + e] = w[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void l1_prox_kernel(int64_t n, const float* __restrict__ v,
+                                                      float lam, float* __restrict__ out,
+                                                      const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    out[e] = l1_shrink(v[e], lam);
+}
+
+// ------------------------------------------------------------------ CG --
+
+// r = b - A x0; p = r; partials of b^T b and r^T r.
+__global__ __launch_bounds__(256) void cg_start_kernel(int64_t n, const float* __restrict__ b,
+                                                       const float* __restrict__ Ax,
+                                                       float* __restrict__ r,
+                                                       float* __restrict__ p,
+                                                       double* __restrict__ ws) {
+  __shared__ double sh[4];
+  double bb = 0.0, rr = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const float be = b[e];
+    const float re = __fsub_rn(be, Ax[e]);
+    r[e] = re;
+    p[e] = re;
+    bb += static_cast<double>(be) * be;
+    rr += static_cast<double>(re) * re;
+  }
+  bb = block_sum_f64(bb, sh);
+  if (threadIdx.x == 0) ws[blockIdx.x] = bb;
+  rr = block_sum_f64(rr, sh);
+  if (threadIdx.x == 0) ws[gridDim.x + blockIdx.x] = rr;
+}
+
+__global__ __launch_bounds__(256) void cg_start_fin(int nparts, const double* __restrict__ ws,
+                                                    double* __restrict__ state,
+                                                    int32_t* __restrict__ flags, int maxiter,
+                                                    double tol, double atol) {
+  __shared__ double sh[4];
+  const double bb = all_sum_f64(ws, nparts, sh);
+  const double rr = all_sum_f64(ws + nparts, nparts, sh);
+  for (int k = 1 + threadIdx.x; k <= maxiter; k += 256) flags[k] = 0;
+  if (threadIdx.x == 0) {
+    const double bn = sqrt(bb);
+    const double t = fmax(tol * bn, atol);
+    state[ST_BN] = bn;
+    state[ST_TOL2] = t * t;
+    state[ST_NITER] = 0.0;
+    state[ST_RELRES] = sqrt(rr) / bn;
+    state[ST_NUM] = rr;
+    flags[0] = (maxiter > 0 && rr > t * t) ? 1 : 0;
+  }
+}
+
+// alpha = num_k / p^T A p; x += alpha p; r -= alpha A p; partials of r^T r.
+__global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, int k, int maxiter,
+                                                    float* __restrict__ x, float* __restrict__ r,
+                                                    const float* __restrict__ p,
+                                                    const float* __restrict__ Ap,
+                                                    double* __restrict__ state,
+                                                    const int32_t* __restrict__ flags,
+                                                    const double* __restrict__ pap_parts,
+                                                    int pap_n, double* __restrict__ ws_rr) {
+  if (flags[k] == 0) return;
+  __shared__ double sh[4];
+  const double pap = all_sum_f64(pap_parts, pap_n, sh);
+  const float alpha = static_cast<float>(static_cast<float>(state[ST_NUM + k]) /
+                                         static_cast<float>(pap));
+  double rr = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const float pe = p[e];
+    x[e] = __fadd_rn(x[e], __fmul_rn(alpha, pe));
+    const float re = __fsub_rn(r[e], __fmul_rn(alpha, Ap[e]));
+    r[e] = re;
+    rr += static_cast<double>(re) * re;
+  }
+  rr = block_sum_f64(rr, sh);
+  if (threadIdx.x == 0) {
+    ws_rr[blockIdx.x] = rr;
+    if (blockIdx.x == 0) state[ST_NUM + (maxiter + 1) + k] = pap;
+  }
+}
+
+// beta = num_{k+1} / num_k; p = r + beta p; stopping test for k + 1.
+__global__ __launch_bounds__(256) void cg_p_kernel(int64_t n, int k, int maxiter,
+                                                   const float* __restrict__ r,
+                                                   float* __restrict__ p,
+                                                   double* __restrict__ state,
+                                                   int32_t* __restrict__ flags,
+                                                   const double* __restrict__ ws_rr, int rr_n) {
+  if (flags[k] == 0) return;
+  __shared__ double sh[4];
+  const double rr = all_sum_f64(ws_rr, rr_n, sh);
+  const float num_new = static_cast<float>(rr);
+  const float beta = num_new / static_cast<float>(state[ST_NUM + k]);
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    p[e] = __fadd_rn(r[e], __fmul_rn(beta, p[e]));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    state[ST_NUM + k + 1] = rr;
+    state[ST_NITER] = k + 1;
+    state[ST_RELRES] = sqrt(rr) / state[ST_BN];
+    flags[k + 1] = (k + 1 < maxiter && rr > state[ST_TOL2]) ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------- ADMM --
+
+// One fused pass of the ADMM z/u update for C = FiniteDifference (2D,
+// append=0 or circular) and g = lam * (L21 over the stack axis | L1):
+//   Cx = C x; z_old = z; z = prox_g(Cx + u, lam / rho); u = (u + Cx) - z
+// (scico/optimize/_admm.py:364-378), plus per-workgroup partials of
+// ||C x - z||^2 (primal residual) and g(z) / lam.
+__global__ __launch_bounds__(256) void admm_zu2d_kernel(int M, int N,
+                                                        const float* __restrict__ x,
+                                                        float* __restrict__ z,
+                                                        float* __restrict__ u,
+                                                        float* __restrict__ zold, float thr,
+                                                        int circ, int iso,
+                                                        double* __restrict__ ws_prml,
+                                                        double* __restrict__ ws_g) {
+  __shared__ double sh[4];
+  const int64_t n = static_cast<int64_t>(M) * N;
+  const bool cc = circ != 0;
+  double prml = 0.0, gsum = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int i = static_cast<int>(e / N), j = static_cast<int>(e - static_cast<int64_t>(i) * N);
+    const float xc = x[e];
+    const int ip = (i + 1 < M) ? i + 1 : 0, jp = (j + 1 < N) ? j + 1 : 0;
+    float cx[2];
+    cx[0] = fd::diff_fwd(xc, x[static_cast<int64_t>(ip) * N + j], i, M, cc);
+    cx[1] = fd::diff_fwd(xc, x[static_cast<int64_t>(i) * N + jp], j, N, cc);
+    float v[2], uo[2];
+    for (int c = 0; c < 2; ++c) {
+      const float zc = z[c * n + e];
+      zold[c * n + e] = zc;
+      uo[c] = u[c * n + e];
+      v[c] = __fadd_rn(cx[c], uo[c]);
+    }
+    if (iso) {
+      l21_shrink(v, 2, thr);
+      gsum += sqrt(static_cast<double>(v[0]) * v[0] + static_cast<double>(v[1]) * v[1]);
+    } else {
+      v[0] = l1_shrink(v[0], thr);
+      v[1] = l1_shrink(v[1], thr);
+      gsum += fabs(static_cast<double>(v[0])) + fabs(static_cast<double>(v[1]));
+    }
+    for (int c = 0; c < 2; ++c) {
+      z[c * n + e] = v[c];
+      u[c * n + e] = __fsub_rn(__fadd_rn(uo[c], cx[c]), v[c]);
+      const double d = static_cast<double>(__fsub_rn(cx[c], v[c]));
+      prml += d * d;
+    }
+  }
+  prml = block_sum_f64(prml, sh);
+  if (threadIdx.x == 0) ws_prml[blockIdx.x] = prml;
+  gsum = block_sum_f64(gsum, sh);
+  if (threadIdx.x == 0) ws_g[blockIdx.x] = gsum;
+}
+
+// Partials of ||rho * C^T (z - z_old)||^2 (dual residual, _admm.py:279-295).
+__global__ __launch_bounds__(256) void admm_dual2d_kernel(int M, int N,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ zold,
+                                                          float rho, int circ,
+                                                          double* __restrict__ ws) {
+  __shared__ double sh[4];
+  const int64_t n = static_cast<int64_t>(M) * N;
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int i = static_cast<int>(e / N), j = static_cast<int>(e - static_cast<int64_t>(i) * N);
+    const float t = __fmul_rn(rho, fd::adj2d_diff(z, zold, M, N, i, j, circ != 0));
+    s += static_cast<double>(t) * t;
+  }
+  s = block_sum_f64(s, sh);
+  if (threadIdx.x == 0) ws[blockIdx.x] = s;
+}
+
+// Iteration statistics row (Objective, Prml Rsdl, Dual Rsdl, CG It, CG Res)
+// appended at hist[5 * counter]; counter is a device int.
+__global__ __launch_bounds__(256) void admm_itstat_kernel(
+    const double* __restrict__ res, int nres, const double* __restrict__ g, int ng,
+    const double* __restrict__ prml, int nprml, const double* __restrict__ dual, int ndual,
+    const double* __restrict__ cg_state, double lam, double rho, double* __restrict__ hist,
+    int32_t* __restrict__ counter, int hist_cap) {
+  __shared__ double sh[4];
+  const double r = all_sum_f64(res, nres, sh);
+  const double gs = all_sum_f64(g, ng, sh);
+  const double p = all_sum_f64(prml, nprml, sh);
+  const double d = all_sum_f64(dual, ndual, sh);
+  if (threadIdx.x == 0) {
+    const int it = *counter;
+    if (it < hist_cap) {
+      double* row = hist + 5 * static_cast<int64_t>(it);
+      row[0] = 0.5 * r + lam * gs;
+      row[1] = sqrt(rho * p);
+      row[2] = sqrt(d);
+      row[3] = cg_state ? cg_state[ST_NITER] : 0.0;
+      row[4] = cg_state ? cg_state[ST_RELRES] : 0.0;
+    }
+    *counter = it + 1;
+  }
+}
+
+unsigned elem_grid(int64_t n) {
+  const int64_t b = (n + 1023) / 1024;
+  return static_cast<unsigned>(b < 1 ? 1 : (b > ELEM_BLOCKS ? ELEM_BLOCKS : b));
+}
+
+}  // namespace solver
+}  // namespace scxr
+
+// ------------------------------------------------------------- C ABI --
+
+extern "C" {
+
+int scxr_fd_forward(int32_t ndim, const int32_t* shape, const float* x, float* out,
+                    int32_t circular, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(ndim >= 1 && ndim <= 3 && shape && x && out, SCXR_ERR_ARG, "bad fd arguments");
+  int d[3] = {1, 1, 1};
+  for (int k = 0; k < ndim; ++k) d[3 - ndim + k] = shape[k];
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  SCXR_REQUIRE(n > 0, SCXR_ERR_SHAPE, "empty array");
+  hipLaunchKernelGGL(solver::fd_fwd_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], x, out, circular, pred);
+  SCXR_LAUNCHED("fd forward");
+  return SCXR_OK;
+}
+
+int scxr_fd_adjoint(int32_t ndim, const int32_t* shape, const float* g, float* out,
+                    int32_t circular, float scale, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(ndim >= 1 && ndim <= 3 && shape && g && out, SCXR_ERR_ARG, "bad fd arguments");
+  int d[3] = {1, 1, 1};
+  for (int k = 0; k < ndim; ++k) d[3 - ndim + k] = shape[k];
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  SCXR_REQUIRE(n > 0, SCXR_ERR_SHAPE, "empty array");
+  hipLaunchKernelGGL(solver::fd_adj_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], g, out, circular, scale, pred);
+  SCXR_LAUNCHED("fd adjoint");
+  return SCXR_OK;
+}
+
+int scxr_fd_gram(int32_t ndim, const int32_t* shape, float rho, const float* x, float* out,
+                 int32_t circular, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(ndim >= 1 && ndim <= 3 && shape && x && out, SCXR_ERR_ARG, "bad fd arguments");
+  int d[3] = {1, 1, 1};
+  for (int k = 0; k < ndim; ++k) d[3 - ndim + k] = shape[k];
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  SCXR_REQUIRE(n > 0, SCXR_ERR_SHAPE, "empty array");
+  hipLaunchKernelGGL(solver::fd_gram_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], x, out, circular, rho, pred);
+  SCXR_LAUNCHED("fd gram");
+  return SCXR_OK;
+}
+
+int scxr_l21_prox(int32_t K, int64_t n, const float* v, float lam, float* out,
+                  const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(K >= 1 && K <= solver::MAX_K && n >= 0 && v && out, SCXR_ERR_ARG,
+               "bad l21_prox arguments (K must be in [1, %d])", solver::MAX_K);
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(solver::l21_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), K, n, v, lam, out, pred);
+  SCXR_LAUNCHED("l21 prox");
+  return SCXR_OK;
+}
+
+int scxr_l1_prox(int64_t n, const float* v, float lam, float* out, const int32_t* pred,
+                 void* stream) {
+  SCXR_REQUIRE(n >= 0 && v && out, SCXR_ERR_ARG, "bad l1_prox arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(solver::l1_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), n, v, lam, out, pred);
+  SCXR_LAUNCHED("l1 prox");
+  return SCXR_OK;
+}
+
+int scxr_cg_start(int64_t n, const float* b, const float* Ax, float* r, float* p, double* state,
+                  int32_t* flags, int32_t maxiter, double tol, double atol, double* ws,
+                  void* stream) {
+  SCXR_REQUIRE(n > 0 && b && Ax && r && p && state && flags && ws && maxiter >= 0, SCXR_ERR_ARG,
+               "bad cg_start arguments");
+  const unsigned g = solver::elem_grid(n);
+  hipLaunchKernelGGL(solver::cg_start_kernel, dim3(g), dim3(256), 0, as_stream(stream), n, b, Ax,
+                     r, p, ws);
+  SCXR_LAUNCHED("cg start");
+  hipLaunchKernelGGL(solver::cg_start_fin, dim3(1), dim3(256), 0, as_stream(stream),
+                     static_cast<int>(g), ws, state, flags, maxiter, tol, atol);
+  SCXR_LAUNCHED("cg start fin");
+  return SCXR_OK;
+}
+
+int scxr_cg_iter(int64_t n, int32_t k, int32_t maxiter, float* x, float* r, float* p,
+                 const float* Ap, double* state, int32_t* flags, const double* pap_parts,
+                 int32_t pap_n, double* ws, void* stream) {
+  SCXR_REQUIRE(n > 0 && x && r && p && Ap && state && flags && pap_parts && ws, SCXR_ERR_ARG,
+               "bad cg_iter arguments");
+  SCXR_REQUIRE(k >= 0 && k < maxiter && pap_n >= 1, SCXR_ERR_ARG, "bad cg_iter index");
+  const unsigned g = solver::elem_grid(n);
+  hipLaunchKernelGGL(solver::cg_xr_kernel, dim3(g), dim3(256), 0, as_stream(stream), n, k,
+                     maxiter, x, r, p, Ap, state, flags, pap_parts, pap_n, ws);
+  SCXR_LAUNCHED("cg xr");
+  hipLaunchKernelGGL(solver::cg_p_kernel, dim3(g), dim3(256), 0, as_stream(stream), n, k, maxiter,
+                     r, p, state, flags, ws, static_cast<int>(g));
+  SCXR_LAUNCHED("cg p");
+  return SCXR_OK;
+}
+
+int scxr_admm_zu2d(int32_t M, int32_t N, const float* x, float* z, float* u, float* zold,
+                   float thr, int32_t circular, int32_t isotropic, double* ws_prml,
+                   double* ws_g, void* stream) {
+  SCXR_REQUIRE(M > 0 && N > 0 && x && z && u && zold && ws_prml && ws_g, SCXR_ERR_ARG,
+               "bad admm_zu2d arguments");
+  const int64_t n = static_cast<int64_t>(M) * N;
+  hipLaunchKernelGGL(solver::admm_zu2d_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), M, N, x, z, u, zold, thr, circular, isotropic, ws_prml,
+                     ws_g);
+  SCXR_LAUNCHED("admm zu2d");
+  return SCXR_OK;
+}
+
+int scxr_admm_dual2d(int32_t M, int32_t N, const float* z, const float* zold, float rho,
+                     int32_t circular, double* ws, void* stream) {
+  SCXR_REQUIRE(M > 0 && N > 0 && z && zold && ws, SCXR_ERR_ARG, "bad admm_dual2d arguments");
+  const int64_t n = static_cast<int64_t>(M) * N;
+  hipLaunchKernelGGL(solver::admm_dual2d_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), M, N, z, zold, rho, circular, ws);
+  SCXR_LAUNCHED("admm dual2d");
+  return SCXR_OK;
+}
+
+int scxr_admm_itstat(const double* res, int32_t nres, const double* g, int32_t ng,
+                     const double* prml, int32_t nprml, const double* dual, int32_t ndual,
+                     const double* cg_state, double lam, double rho, double* hist,
+                     int32_t* counter, int32_t hist_cap, void* stream) {
+  SCXR_REQUIRE(res && g && prml && dual && hist && counter, SCXR_ERR_ARG,
+               "bad admm_itstat arguments");
+  hipLaunchKernelGGL(solver::admm_itstat_kernel, dim3(1), dim3(256), 0, as_stream(stream), res,
+                     nres, g, ng, prml, nprml, dual, ndual, cg_state, lam, rho, hist, counter,
+                     hist_cap);
+  SCXR_LAUNCHED("admm itstat");
+  return SCXR_OK;
+}
+
+int32_t scxr_elem_blocks(int64_t n) { return static_cast<int32_t>(solver::elem_grid(n)); }
+
+}  // extern "C"

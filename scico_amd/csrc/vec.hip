@@ -79,18 +79,6 @@ __device__ __forceinline__ double red_term(int kind, const float* x, const float
   }
 }
 
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  // wave reduction then across the 4 waves, fixed order
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-  __syncthreads();
-  return r;
-}
-
 __global__ __launch_bounds__(256) void reduce_pass1(int kind, int64_t n, int K,
                                                     const float* __restrict__ x,
                                                     const float* __restrict__ y,
@@ -101,7 +89,7 @@ __global__ __launch_bounds__(256) void reduce_pass1(int kind, int64_t n, int K,
   double s = 0.0;
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
     s += red_term(kind, x, y, e, n, K);
-  s = block_sum(s, sh);
+  s = block_sum_f64(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
@@ -113,7 +101,7 @@ __global__ __launch_bounds__(256) void reduce_pass2(int nparts, const double* __
   __shared__ double sh[4];
   double s = 0.0;
   for (int e = threadIdx.x; e < nparts; e += 256) s += partial[e];
-  s = block_sum(s, sh);
+  s = block_sum_f64(s, sh);
   if (threadIdx.x == 0) {
     s *= scale;
     if (post == POST_SQRT) s = sqrt(s);

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fd_impl.h"
 #include "xray2d_impl.h"
 
 using namespace scxr;
@@ -91,7 +92,18 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
     const std::vector<int>& list = cls ? cols : rows;
     for (size_t s = 0; s < list.size(); s += p.vpg) {
       const int cnt = static_cast<int>(std::min<size_t>(p.vpg, list.size() - s));
-      groups.push_back(make_int4(static_cast<int>(order.size()), cnt, cls, 0));
+      // Lane stride along the fast axis: S * |P_fast| >= 2 (+ margin for
+      // the float32 rounding of Px) keeps the lanes' footprints disjoint.
+      int S = 1;
+      for (int k = 0; k < cnt; ++k) {
+        const float4 g = geom[list[s + k]];
+        const double pf = std::max(std::fabs(static_cast<double>(cls ? g.y : g.z)), 1e-30);
+        S = std::max(S, static_cast<int>(std::ceil(2.0625 / pf)));
+      }
+      SCXR_REQUIRE(S <= FWD_SMAX, SCXR_ERR_GEOM,
+                   "projected pixel spacing too small for the forward projector "
+                   "(lane stride %d > %d)", S, FWD_SMAX);
+      groups.push_back(make_int4(static_cast<int>(order.size()), cnt, cls, S));
       for (int k = 0; k < cnt; ++k) order.push_back(list[s + k]);
     }
   }
@@ -99,21 +111,29 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
 
   // Adjoint: detector window per (tile, view) = all bins an ADJ_TI x ADJ_TJ
   // pixel tile can reach, plus margin for rounding.
+  // Window origin offset per view (see adj_kernel): a lower bound of
+  // floor(Px) - 1 over any tile; the window spans the projected tile
+  // extent plus the lower margin, the rounding of Px and bin ind + 1.
+  std::vector<float> clo(A);
   int wmax = 0;
   for (int a = 0; a < A; ++a) {
-    const double ext = (ADJ_TI - 1) * std::fabs(static_cast<double>(geom[a].y)) +
-                       (ADJ_TJ - 1) * std::fabs(static_cast<double>(geom[a].z));
-    wmax = std::max(wmax, static_cast<int>(std::ceil(ext)) + 4);
+    const double P0 = geom[a].y, P1 = geom[a].z;
+    clo[a] = static_cast<float>(static_cast<double>(geom[a].x) +
+                                std::min(P0, 0.0) * (ADJ_TI - 1) +
+                                std::min(P1, 0.0) * (ADJ_TJ - 1) - 1.0);
+    const double ext = (ADJ_TI - 1) * std::fabs(P0) + (ADJ_TJ - 1) * std::fabs(P1);
+    wmax = std::max(wmax, static_cast<int>(std::ceil(ext)) + 5);
   }
-  p.adj_w = (wmax + 3) & ~3;
-  SCXR_REQUIRE(p.adj_w <= ADJ_WMAX, SCXR_ERR_GEOM,
+  p.adj_w = wmax;
+  SCXR_REQUIRE(p.adj_w <= ADJ_WFAST, SCXR_ERR_GEOM,
                "projected pixel footprint too large for the adjoint tile (%d bins > %d); "
                "pixel size dx is too large relative to the detector spacing",
-               p.adj_w, ADJ_WMAX);
+               p.adj_w, ADJ_WFAST);
 
   size_t bytes_geom = sizeof(float4) * A, bytes_rw = sizeof(double) * A;
   size_t bytes_groups = sizeof(int4) * groups.size(), bytes_order = sizeof(int) * A;
-  size_t total = bytes_geom + bytes_rw + bytes_groups + bytes_order + 256 * 4;
+  size_t bytes_clo = sizeof(float) * A;
+  size_t total = bytes_geom + bytes_rw + bytes_groups + bytes_order + bytes_clo + 256 * 5;
   if (hipMalloc(&p.dev_block, total) != hipSuccess)
     return fail(SCXR_ERR_NOMEM, "hipMalloc of %zu bytes for the plan tables failed", total);
   char* base = static_cast<char*>(p.dev_block);
@@ -126,6 +146,8 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   p.rw = reinterpret_cast<double*>(carve(bytes_rw));
   p.groups = reinterpret_cast<int4*>(carve(bytes_groups));
   p.order = reinterpret_cast<int*>(carve(bytes_order));
+  p.clo = reinterpret_cast<float*>(carve(bytes_clo));
+  SCXR_HIP(hipMemcpy(p.clo, clo.data(), bytes_clo, hipMemcpyHostToDevice));
   SCXR_HIP(hipMemcpy(p.geom, geom.data(), bytes_geom, hipMemcpyHostToDevice));
   SCXR_HIP(hipMemcpy(p.rw, rw.data(), bytes_rw, hipMemcpyHostToDevice));
   SCXR_HIP(hipMemcpy(p.groups, groups.data(), bytes_groups, hipMemcpyHostToDevice));
@@ -152,12 +174,16 @@ struct Weight {
   float w;   // weight of bin fl; bin fl + 1 gets 1 - w
 };
 
+// The reference computes RN(min(dd, width) / width) with dd = 1 - (Px - ind).
+// Here the quotient is clamped instead: if dd < width both forms give
+// RN(dd / width); if dd >= width, RN(dd * rw) >= 1 and both give 1.0.
 __device__ __forceinline__ Weight weight_at(float Pxmin, float P0, float P1, float width,
                                             double rw, float fi, float fj) {
+  (void)width;
   const float Px = __fadd_rn(__fadd_rn(Pxmin, __fmul_rn(P0, fi)), __fmul_rn(P1, fj));
   const float fl = floorf(Px);
-  const float d = fminf(__fsub_rn(1.0f, __fsub_rn(Px, fl)), width);
-  return Weight{fl, div_rn(d, rw)};
+  const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
+  return Weight{fl, fminf(div_rn(dd, rw), 1.0f)};
 }
 
 // Transpose (M, N) -> (N, M) through a padded LDS tile.
@@ -180,37 +206,92 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 }
 
 // Forward projection.  One workgroup owns up to V whole sinogram rows
-// (views) and streams the entire image; each of its 4 waves accumulates
-// into a private LDS copy of every owned row with LDS float atomics, and
-// the copies are summed in a fixed order at the end, so the result is
-// deterministic and needs no global atomics.  Row slot k holds detector
-// bin k - 2; bins outside [0, D) land in the two guard slots on each side.
+// (views) and streams the entire image.  Each of its 4 waves accumulates
+// into a private LDS copy of every owned row, and the copies are summed in
+// a fixed order at the end: deterministic, no global atomics.  Row slot k
+// holds detector bin k - 2; bins outside [0, D) land in two guard slots on
+// each side.
+//
+// No LDS atomics either: within one instruction the 64 lanes take pixels S
+// apart along the fast image axis, where S * |P_fast| >= 2 (S is chosen on
+// the host per group of views).  Two such pixels project at least two bins
+// apart, so the lanes' two-bin footprints {ind, ind + 1} are disjoint and a
+// plain LDS read-modify-write is race free.  The S phases of a wave run in
+// program order; LDS executes a wave's accesses in order, and a compiler
+// fence keeps the phases' memory operations in that order.
+// Each wave runs two independent chains (lines s and s + 4, each with its
+// own private row copy) so that one chain's LDS round trip overlaps the
+// other's arithmetic.  Px = (Pxmin + P0 * i) + P1 * j is evaluated in the
+// reference order with the line-constant product hoisted: for row-fast
+// views t = Pxmin + P0 * i is constant along the line, for column-fast
+// views t = P1 * j is.
 template <int V, bool COLFAST>
 __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* acc, int L,
-                                         int cnt, const float* Pxmin, const float* P0,
-                                         const float* P1, const float* wd, const double* rw,
-                                         int M, int N, int D) {
+                                         int cnt, int S, const float* Pxmin, const float* P0,
+                                         const float* P1, const double* rw, int M, int N,
+                                         int D) {
+  static_assert(FWD_CH == 2, "two chains per wave");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int Lf = COLFAST ? M : N;  // fast (lane) axis length
+  const unsigned Lf = COLFAST ? M : N;  // fast (lane) axis length
   const int Ls = COLFAST ? N : M;
   const float Dmax = static_cast<float>(D);
-  for (int s = wave; s < Ls; s += 4) {
-    const float fs = static_cast<float>(s);
-    const float* srow = src + static_cast<size_t>(s) * Lf;
-    for (int f = lane; f < Lf; f += 64) {
-      const float val = srow[f];
-      const float ff = static_cast<float>(f);
-      const float fi = COLFAST ? ff : fs;
-      const float fj = COLFAST ? fs : ff;
+  const unsigned span = 64u * S;
+  const unsigned loff = static_cast<unsigned>(S * lane);
+  int rowa[V], rowb[V];
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        if (v < cnt) {
-          const Weight g = weight_at(Pxmin[v], P0[v], P1[v], wd[v], rw[v], fi, fj);
-          const int k = static_cast<int>(fminf(fmaxf(g.fl, -2.0f), Dmax)) + 2;
-          float* row = acc + (v * 4 + wave) * L;
-          atomicAdd(row + k, __fmul_rn(val, g.w));
-          atomicAdd(row + k + 1, __fmul_rn(val, __fsub_rn(1.0f, g.w)));
+  for (int v = 0; v < V; ++v) {
+    rowa[v] = (v * 4 + wave) * 2 * L + 2;
+    rowb[v] = rowa[v] + L;
+  }
+  for (int sa = wave; sa < Ls; sa += 8) {
+    const int sb = sa + 4;
+    const bool has_b = sb < Ls;
+    const float* __restrict__ ra = src + static_cast<size_t>(sa) * Lf;
+    const float* __restrict__ rb = src + static_cast<size_t>(has_b ? sb : sa) * Lf;
+    const float scale_b = has_b ? 1.0f : 0.0f;
+    float ta[V], tb[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const float fa = static_cast<float>(sa), fb = static_cast<float>(sb);
+      ta[v] = COLFAST ? __fmul_rn(P1[v], fa) : __fadd_rn(Pxmin[v], __fmul_rn(P0[v], fa));
+      tb[v] = COLFAST ? __fmul_rn(P1[v], fb) : __fadd_rn(Pxmin[v], __fmul_rn(P0[v], fb));
+    }
+    for (unsigned c0 = 0; c0 < Lf; c0 += span) {
+      unsigned f = c0 + loff;
+      float ff = static_cast<float>(f);
+      for (int r = 0; r < S; ++r, ++f, ff += 1.0f) {
+        if (f < Lf) {
+          const float va = ra[f];
+          const float vb = __fmul_rn(rb[f], scale_b);
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (V == 1 || v < cnt) {
+              float Pa, Pb;
+              if (COLFAST) {
+                const float t = __fadd_rn(Pxmin[v], __fmul_rn(P0[v], ff));
+                Pa = __fadd_rn(t, ta[v]);
+                Pb = __fadd_rn(t, tb[v]);
+              } else {
+                const float t = __fmul_rn(P1[v], ff);
+                Pa = __fadd_rn(ta[v], t);
+                Pb = __fadd_rn(tb[v], t);
+              }
+              const float fla = floorf(Pa), flb = floorf(Pb);
+              const float wa = fminf(div_rn(__fsub_rn(1.0f, __fsub_rn(Pa, fla)), rw[v]), 1.0f);
+              const float wb = fminf(div_rn(__fsub_rn(1.0f, __fsub_rn(Pb, flb)), rw[v]), 1.0f);
+              const int ka = rowa[v] + static_cast<int>(__builtin_amdgcn_fmed3f(fla, -2.0f, Dmax));
+              const int kb = rowb[v] + static_cast<int>(__builtin_amdgcn_fmed3f(flb, -2.0f, Dmax));
+              const float a0 = acc[ka], a1 = acc[ka + 1];
+              const float b0 = acc[kb], b1 = acc[kb + 1];
+              acc[ka] = __fadd_rn(a0, __fmul_rn(va, wa));
+              acc[ka + 1] = __fadd_rn(a1, __fmul_rn(va, __fsub_rn(1.0f, wa)));
+              acc[kb] = __fadd_rn(b0, __fmul_rn(vb, wb));
+              acc[kb + 1] = __fadd_rn(b1, __fmul_rn(vb, __fsub_rn(1.0f, wb)));
+            }
+          }
         }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_wave_barrier();
       }
     }
   }
@@ -224,9 +305,12 @@ __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
                                                   const double* __restrict__ rwt,
                                                   const int4* __restrict__ groups,
                                                   const int* __restrict__ order, int M, int N,
-                                                  int D, const int32_t* pred) {
+                                                  int D, const float* __restrict__ yres,
+                                                  double* __restrict__ res_ws,
+                                                  const int32_t* pred) {
   if (pred_off(pred)) return;
-  extern __shared__ float acc[];  // [V][4 waves][L]
+  extern __shared__ float acc[];  // [V][4 waves][FWD_CH chains][L]
+  __shared__ double red[4];
   const int L = D + 4;
   const int4 grp = groups[blockIdx.x];
   const int cnt = grp.y;
@@ -244,80 +328,174 @@ __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
     wd[v] = g.w;
     rw[v] = rwt[a];
   }
-  for (int e = threadIdx.x; e < V * 4 * L; e += 256) acc[e] = 0.0f;
+  for (int e = threadIdx.x; e < V * 4 * FWD_CH * L; e += 256) acc[e] = 0.0f;
   __syncthreads();
-  if (grp.z)
-    fwd_body<V, true>(imT, acc, L, cnt, Pxmin, P0, P1, wd, rw, M, N, D);
+  if (grp.z & 1)
+    fwd_body<V, true>(imT, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
   else
-    fwd_body<V, false>(im, acc, L, cnt, Pxmin, P0, P1, wd, rw, M, N, D);
+    fwd_body<V, false>(im, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
   __syncthreads();
+  double res = 0.0;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     if (v < cnt) {
-      const float* r = acc + v * 4 * L + 2;
+      const float* r = acc + v * 4 * FWD_CH * L + 2;
       float* dst = sino + static_cast<size_t>(view[v]) * D;
-      for (int b = threadIdx.x; b < D; b += 256)
-        dst[b] = __fadd_rn(__fadd_rn(__fadd_rn(r[b], r[L + b]), r[2 * L + b]), r[3 * L + b]);
+      const float* yrow = yres ? yres + static_cast<size_t>(view[v]) * D : nullptr;
+      for (int b = threadIdx.x; b < D; b += 256) {
+        float val = r[b];
+        for (int c = 1; c < 4 * FWD_CH; ++c) val = __fadd_rn(val, r[c * L + b]);
+        dst[b] = val;
+        if (yrow) {
+          const double d = static_cast<double>(__fsub_rn(yrow[b], val));
+          res += d * d;
+        }
+      }
     }
+  }
+  if (res_ws) {  // per-workgroup partial of sum (y - A x)^2
+    res = block_sum_f64(res, red);
+    if (threadIdx.x == 0) res_ws[blockIdx.x] = res;
   }
 }
 
 // Adjoint (back projection).  One workgroup per ADJ_TI x ADJ_TJ pixel
-// tile; views are processed in chunks of ADJ_VC: for each view the
-// detector window reachable from the tile is staged in LDS (zero outside
-// [0, D)), then every thread gathers its pixel's two bins.  The two sums
+// tile; views are processed in chunks of ADJ_VC.  For each view the
+// detector window reachable from the tile (W <= 64 bins, zero outside
+// [0, D)) and the view's geometry are staged in LDS, then every thread
+// gathers its pixel's two bins.  The windows of the next chunk are
+// prefetched into registers while the current chunk is computed, so the
+// global-load latency is hidden behind the gather arithmetic.  The two sums
 // of _xray2d.py:299-304 are kept separately and added at the end.
+//
+// Window origin of view a for the tile at (i0, j0):
+//   lo = floor(P0 * i0 + P1 * j0 + clo[a]),
+//   clo[a] = Pxmin + min(P0, 0) (TI - 1) + min(P1, 0) (TJ - 1) - 1,
+// a lower bound of floor(Px) over the tile with a one-bin margin for the
+// float32 rounding of Px; W covers the tile's projected extent plus the
+// margins, so every pixel's bins ind and ind + 1 lie inside the window.
+struct __align__(16) ViewLds {
+  float Pxmin, P0, P1, pad0;
+  double rw;
+  int base;  // index in win[] of detector bin 0 for this view
+  int pad1;
+};
+
 __global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino,
                                                   float* __restrict__ out,
                                                   const float4* __restrict__ geom,
-                                                  const double* __restrict__ rwt, int M, int N,
-                                                  int A, int D, int W, const int32_t* pred) {
+                                                  const double* __restrict__ rwt,
+                                                  const float* __restrict__ clo, int M, int N,
+                                                  int A, int D, int W, AdjEpi epi,
+                                                  const int32_t* pred) {
   if (pred_off(pred)) return;
-  extern __shared__ float smem[];
-  float* win = smem;                                    // [ADJ_VC][W]
-  float* lo_s = smem + ADJ_VC * W;                      // [ADJ_VC]
+  constexpr int VPW = ADJ_VC / 4;  // views staged per wave
+  __shared__ float win[ADJ_VC * ADJ_WFAST];
+  __shared__ ViewLds gv[ADJ_VC];
+  __shared__ double red[4];
   const int tx = threadIdx.x % ADJ_TJ, ty = threadIdx.x / ADJ_TJ;
   const int i0 = blockIdx.y * ADJ_TI, j0 = blockIdx.x * ADJ_TJ;
   const int i = i0 + ty, j = j0 + tx;
   const bool inside = (i < M) && (j < N);
   const float fi = static_cast<float>(i), fj = static_cast<float>(j);
-  const float ci0 = static_cast<float>(i0), ci1 = static_cast<float>(min(i0 + ADJ_TI, M) - 1);
-  const float cj0 = static_cast<float>(j0), cj1 = static_cast<float>(min(j0 + ADJ_TJ, N) - 1);
+  const float fi0 = static_cast<float>(i0), fj0 = static_cast<float>(j0);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float kmax = static_cast<float>(W - 2);
+
+  float pf[VPW];
+  int lop[VPW];
+  auto prefetch = [&](int a0) {
+#pragma unroll
+    for (int q = 0; q < VPW; ++q) {
+      const int a = a0 + wave + 4 * q;
+      float val = 0.0f;
+      int lo = 0;
+      if (a < A) {
+        const float4 g = geom[a];
+        lo = static_cast<int>(floorf(fmaf(g.y, fi0, fmaf(g.z, fj0, clo[a]))));
+        const int b = lo + lane;
+        if (lane < W && static_cast<unsigned>(b) < static_cast<unsigned>(D))
+          val = sino[a * D + b];
+      }
+      pf[q] = val;
+      lop[q] = lo;
+    }
+  };
+
   float acc0 = 0.0f, acc1 = 0.0f;
+  prefetch(0);
   for (int a0 = 0; a0 < A; a0 += ADJ_VC) {
     const int nv = min(ADJ_VC, A - a0);
-    __syncthreads();
-    for (int v = wave; v < nv; v += 4) {
-      const float4 g = geom[a0 + v];
-      const float r0 = __fadd_rn(g.x, __fmul_rn(g.y, ci0));
-      const float r1 = __fadd_rn(g.x, __fmul_rn(g.y, ci1));
-      const float c0 = __fmul_rn(g.z, cj0), c1 = __fmul_rn(g.z, cj1);
-      const float pmin = fminf(fminf(__fadd_rn(r0, c0), __fadd_rn(r0, c1)),
-                               fminf(__fadd_rn(r1, c0), __fadd_rn(r1, c1)));
-      const float lof = floorf(pmin);
-      const int lo = static_cast<int>(lof);
-      const float* srow = sino + static_cast<size_t>(a0 + v) * D;
-      for (int k = lane; k < W; k += 64) {
-        const int b = lo + k;
-        win[v * W + k] = (b >= 0 && b < D) ? srow[b] : 0.0f;
+    const int nv4 = (nv + 3) & ~3;  // padded views: zero window, benign geometry
+    __syncthreads();  // the previous chunk's gathers are done
+#pragma unroll
+    for (int q = 0; q < VPW; ++q) {
+      const int v = wave + 4 * q;
+      if (v < nv4) {
+        const bool real = v < nv;
+        if (lane < W) win[v * W + lane] = real ? pf[q] : 0.0f;
+        if (lane == 0) {
+          ViewLds t;
+          if (real) {
+            const float4 g = geom[a0 + v];
+            t.Pxmin = g.x;
+            t.P0 = g.y;
+            t.P1 = g.z;
+            t.rw = rwt[a0 + v];
+            t.base = v * W - lop[q];
+          } else {
+            t.Pxmin = t.P0 = t.P1 = 0.0f;
+            t.rw = 1.0;
+            t.base = v * W;
+          }
+          t.pad0 = 0.0f;
+          t.pad1 = 0;
+          gv[v] = t;
+        }
       }
-      if (lane == 0) lo_s[v] = lof;
     }
     __syncthreads();
+    if (a0 + ADJ_VC < A) prefetch(a0 + ADJ_VC);
     if (inside) {
-      for (int v = 0; v < nv; ++v) {
-        const float4 g = geom[a0 + v];
-        const Weight wt = weight_at(g.x, g.y, g.z, g.w, rwt[a0 + v], fi, fj);
-        const int k = static_cast<int>(fminf(fmaxf(__fsub_rn(wt.fl, lo_s[v]), 0.0f), kmax));
-        const float* wrow = win + v * W + k;
-        acc0 = fmaf(wrow[0], wt.w, acc0);
-        acc1 = fmaf(wrow[1], __fsub_rn(1.0f, wt.w), acc1);
+      for (int v = 0; v < nv4; v += 4) {
+        float w[4];
+        int k[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const ViewLds t = gv[v + u];
+          const Weight wt = weight_at(t.Pxmin, t.P0, t.P1, 0.0f, t.rw, fi, fj);
+          w[u] = wt.w;
+          k[u] = static_cast<int>(wt.fl) + t.base;
+        }
+        float y0[4], y1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          y0[u] = win[k[u]];
+          y1[u] = win[k[u] + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          acc0 = fmaf(y0[u], w[u], acc0);
+          acc1 = fmaf(y1[u], __fsub_rn(1.0f, w[u]), acc1);
+        }
       }
     }
   }
-  if (inside) out[static_cast<size_t>(i) * N + j] = __fadd_rn(acc0, acc1);
+  double dot = 0.0;
+  if (inside) {
+    float v = __fadd_rn(acc0, acc1);
+    const bool circ = epi.circular != 0;
+    if (epi.kind == EPI_GRAM) {  // (rho * C^T C)(p) + A^T A p   (_admmaux.py:222-229)
+      v = __fadd_rn(__fmul_rn(epi.rho, fd::gram2d(epi.p, M, N, i, j, circ)), v);
+    } else if (epi.kind == EPI_RHS) {  // A^T y + rho * C^T (z - u)   (_admmaux.py:245-256)
+      v = __fadd_rn(v, __fmul_rn(epi.rho, fd::adj2d_diff(epi.z, epi.u, M, N, i, j, circ)));
+    }
+    out[static_cast<size_t>(i) * N + j] = v;
+    if (epi.dot_ws) dot = static_cast<double>(epi.p[static_cast<size_t>(i) * N + j]) * v;
+  }
+  if (epi.dot_ws) {  // per-workgroup partial of sum p * out (CG curvature p^T A p)
+    dot = block_sum_f64(dot, red);
+    if (threadIdx.x == 0) epi.dot_ws[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+  }
 }
 
 // Debug dump of (ind, weight) for every (view, pixel).
@@ -339,7 +517,8 @@ __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds
   wts[e] = wt.w;
 }
 
-int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s) {
+int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,
+            const float* yres, double* res_ws) {
   if (p.any_colfast) {
     dim3 g(ceil_div(p.N, 32), ceil_div(p.M, 32));
     hipLaunchKernelGGL(transpose_kernel, g, dim3(256), 0, s, im, p.work, p.M, p.N, pred);
@@ -349,26 +528,26 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
   switch (p.vpg) {
     case 1:
       hipLaunchKernelGGL(fwd_kernel<1>, dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino,
-                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, pred);
+                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred);
       break;
     case 2:
       hipLaunchKernelGGL(fwd_kernel<2>, dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino,
-                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, pred);
+                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred);
       break;
     default:
       hipLaunchKernelGGL(fwd_kernel<4>, dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino,
-                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, pred);
+                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred);
       break;
   }
   SCXR_LAUNCHED("xr2d forward");
   return SCXR_OK;
 }
 
-int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s) {
+int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,
+            const AdjEpi& epi) {
   dim3 g(ceil_div(p.N, ADJ_TJ), ceil_div(p.M, ADJ_TI));
-  const size_t lds = sizeof(float) * (static_cast<size_t>(ADJ_VC) * p.adj_w + ADJ_VC);
-  hipLaunchKernelGGL(adj_kernel, g, dim3(256), lds, s, sino, im, p.geom, p.rw, p.M, p.N, p.A,
-                     p.D, p.adj_w, pred);
+  hipLaunchKernelGGL(adj_kernel, g, dim3(256), 0, s, sino, im, p.geom, p.rw, p.clo, p.M, p.N,
+                     p.A, p.D, p.adj_w, epi, pred);
   SCXR_LAUNCHED("xr2d adjoint");
   return SCXR_OK;
 }
@@ -377,6 +556,8 @@ int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hi
 }  // namespace scxr
 
 // ------------------------------------------------------------- C ABI --
+
+using namespace scxr::xr2d;
 
 extern "C" {
 
@@ -420,13 +601,50 @@ int scxr_xr2d_view_table(const scxr_xr2d_plan* plan, float* out) {
 int scxr_xr2d_forward(const scxr_xr2d_plan* plan, const float* im, float* sino,
                       const int32_t* pred, void* stream) {
   SCXR_REQUIRE(plan && im && sino, SCXR_ERR_ARG, "null argument to xr2d_forward");
-  return xr2d::forward(plan->p, im, sino, pred, as_stream(stream));
+  return xr2d::forward(plan->p, im, sino, pred, as_stream(stream), nullptr, nullptr);
+}
+
+int scxr_xr2d_forward_res(const scxr_xr2d_plan* plan, const float* im, float* sino,
+                          const float* y, double* res_ws, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(plan && im && sino && y && res_ws, SCXR_ERR_ARG,
+               "null argument to xr2d_forward_res");
+  return xr2d::forward(plan->p, im, sino, pred, as_stream(stream), y, res_ws);
 }
 
 int scxr_xr2d_adjoint(const scxr_xr2d_plan* plan, const float* sino, float* im,
                       const int32_t* pred, void* stream) {
   SCXR_REQUIRE(plan && im && sino, SCXR_ERR_ARG, "null argument to xr2d_adjoint");
-  return xr2d::adjoint(plan->p, sino, im, pred, as_stream(stream));
+  return xr2d::adjoint(plan->p, sino, im, pred, as_stream(stream), AdjEpi{});
+}
+
+int scxr_xr2d_adjoint_epi(const scxr_xr2d_plan* plan, const float* sino, float* im,
+                          int32_t kind, float rho, int32_t circular, const float* p,
+                          const float* z, const float* u, double* dot_ws, const int32_t* pred,
+                          void* stream) {
+  SCXR_REQUIRE(plan && im && sino, SCXR_ERR_ARG, "null argument to xr2d_adjoint_epi");
+  SCXR_REQUIRE(kind >= EPI_NONE && kind <= EPI_RHS, SCXR_ERR_ARG, "bad epilogue kind %d", kind);
+  SCXR_REQUIRE(kind != EPI_GRAM || p, SCXR_ERR_ARG, "gram epilogue needs p");
+  SCXR_REQUIRE(kind != EPI_RHS || z, SCXR_ERR_ARG, "rhs epilogue needs z");
+  SCXR_REQUIRE(!dot_ws || p, SCXR_ERR_ARG, "dot partials need p");
+  AdjEpi e;
+  e.kind = kind;
+  e.circular = circular;
+  e.rho = rho;
+  e.p = p;
+  e.z = z;
+  e.u = u;
+  e.dot_ws = dot_ws;
+  return xr2d::adjoint(plan->p, sino, im, pred, as_stream(stream), e);
+}
+
+int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out) {
+  SCXR_REQUIRE(plan && out, SCXR_ERR_ARG, "null argument to xr2d_info");
+  const auto& p = plan->p;
+  out[0] = p.n_groups;
+  out[1] = static_cast<int32_t>(ceil_div(p.N, ADJ_TJ) * ceil_div(p.M, ADJ_TI));
+  out[2] = p.vpg;
+  out[3] = p.adj_w;
+  return SCXR_OK;
 }
 
 int scxr_xr2d_weights(const scxr_xr2d_plan* plan, int32_t* inds, float* weights, void* stream) {

@@ -10,16 +10,19 @@ namespace scxr {
 namespace xr2d {
 
 // Adjoint tiling: ADJ_TI x ADJ_TJ pixels per workgroup (one per thread),
-// ADJ_VC views staged per LDS chunk, at most ADJ_WMAX bins per window.
+// ADJ_VC views staged per LDS chunk, at most ADJ_WFAST bins per window.
 constexpr int ADJ_TI = 16;
 constexpr int ADJ_TJ = 16;
 constexpr int ADJ_VC = 32;
-constexpr int ADJ_WMAX = 256;
+constexpr int ADJ_WFAST = 64;  // window slots (one per lane)
 
-// Forward: 4 private LDS copies of (D + 4) bins per owned view.
+// Forward: 4 waves x FWD_CH chains private LDS copies of (D + 4) bins per
+// owned view.
+constexpr int FWD_CH = 2;
 constexpr size_t FWD_LDS_MAX = 64 * 1024;
+constexpr int FWD_SMAX = 64;  // largest lane stride of the forward projector
 inline size_t fwd_lds_bytes(int vpg, int D) {
-  return sizeof(float) * static_cast<size_t>(vpg) * 4 * (static_cast<size_t>(D) + 4);
+  return sizeof(float) * static_cast<size_t>(vpg) * 4 * FWD_CH * (static_cast<size_t>(D) + 4);
 }
 
 struct Plan {
@@ -33,8 +36,21 @@ struct Plan {
   double* rw = nullptr;       // RN64(1 / width) per view
   int4* groups = nullptr;     // forward groups: (first, count, class, 0)
   int* order = nullptr;       // views grouped by class
+  float* clo = nullptr;       // adjoint window origin offset per view
   float* work = nullptr;      // transposed image workspace (M * N)
   std::vector<float4> host_geom;
+};
+
+// Adjoint epilogues (applied per pixel after the back projection sum).
+enum { EPI_NONE = 0, EPI_GRAM = 1, EPI_RHS = 2 };
+struct AdjEpi {
+  int kind = EPI_NONE;
+  int circular = 0;
+  float rho = 0.0f;
+  const float* p = nullptr;  // EPI_GRAM: image whose FD gram is added; dot partner
+  const float* z = nullptr;  // EPI_RHS: (2, M, N)
+  const float* u = nullptr;  // EPI_RHS: (2, M, N), may be null
+  double* dot_ws = nullptr;  // per-workgroup partials of sum p * out (nullable)
 };
 
 void view_geometry(int A, const float* angles, const double* x0, const double* dx, double y0,
@@ -42,8 +58,10 @@ void view_geometry(int A, const float* angles, const double* x0, const double* d
 int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx, double y0,
            const float* angles);
 void destroy(Plan& p);
-int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s);
-int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s);
+int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,
+            const float* yres, double* res_ws);
+int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,
+            const AdjEpi& epi);
 
 }  // namespace xr2d
 }  // namespace scxr

@@ -1,5 +1,6 @@
 """Linear operators (``scico.linop`` surface for the hot path)."""
 
+from ._diff import FiniteDifference
 from ._linop import ComposedLinearOperator, Identity, LinearOperator, valid_adjoint
 
-__all__ = ["LinearOperator", "ComposedLinearOperator", "Identity", "valid_adjoint"]
+__all__ = ["FiniteDifference", "LinearOperator", "ComposedLinearOperator", "Identity", "valid_adjoint"]

@@ -16,6 +16,7 @@ from warnings import warn
 import numpy as np
 
 from ... import _device as dev
+from ... import _pred
 from ..._lib import call, load
 from .._linop import LinearOperator
 
@@ -127,7 +128,7 @@ class XRayTransform2D(LinearOperator):
         """X-ray projection ``H @ im`` (``_xray2d.py:138-146``)."""
         im = dev.as_device(im)
         out = dev.empty(self.output_shape) if out is None else out
-        call("scxr_xr2d_forward", self.plan, im.data_ptr(), out.data_ptr(), dev.ptr(pred),
+        call("scxr_xr2d_forward", self.plan, im.data_ptr(), out.data_ptr(), dev.ptr(_pred.resolve(pred)),
              dev.stream())
         return out
 
@@ -135,7 +136,7 @@ class XRayTransform2D(LinearOperator):
         """Back projection ``H.T @ y`` (``_xray2d.py:148-156``)."""
         y = dev.as_device(y)
         out = dev.empty(self.input_shape) if out is None else out
-        call("scxr_xr2d_adjoint", self.plan, y.data_ptr(), out.data_ptr(), dev.ptr(pred),
+        call("scxr_xr2d_adjoint", self.plan, y.data_ptr(), out.data_ptr(), dev.ptr(_pred.resolve(pred)),
              dev.stream())
         return out
 

@@ -1,0 +1,8 @@
+"""Optimizers (``scico.optimize`` surface for the hot path)."""
+
+from ._admm import ADMM
+from ._admmaux import GenericSubproblemSolver, LinearSubproblemSolver, SubproblemSolver
+from ._common import Optimizer
+
+__all__ = ["ADMM", "LinearSubproblemSolver", "GenericSubproblemSolver", "SubproblemSolver",
+           "Optimizer"]

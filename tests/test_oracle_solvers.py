@@ -1,0 +1,157 @@
+"""Pins for the solver-side oracle (CPU)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+from oracle import solvers as S
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+KAT = json.load(open(os.path.join(GOLD, "reference_kat.json")))
+
+
+def test_fd_reference_outputs():
+    k = KAT["fd_append0_1d"]
+    np.testing.assert_array_equal(S.fd(np.array(k["x"], np.float32))[0], k["out"])
+    k = KAT["fd_2d_doc"]
+    g = S.fd(np.array(k["x"], np.float32))
+    # append=0 adds a trailing zero row/column to the doctest's plain diff
+    np.testing.assert_array_equal(g[0][:-1], k["axis0"])
+    np.testing.assert_array_equal(g[1][:, :-1], k["axis1"])
+    assert np.all(g[0][-1] == 0) and np.all(g[1][:, -1] == 0)
+
+
+@pytest.mark.parametrize("shape", [(7,), (5, 6), (3, 4, 5)])
+@pytest.mark.parametrize("circular", [False, True])
+def test_fd_adjoint(shape, circular):
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(shape).astype(np.float32)
+    g = rng.standard_normal((len(shape), *shape)).astype(np.float32)
+    a = np.vdot(S.fd(x, circular).astype(np.float64), g)
+    b = np.vdot(x.astype(np.float64), S.fd_adj(g, circular))
+    assert abs(a - b) <= 1e-5 * max(abs(a), abs(b))
+    if circular:
+        for ax in range(len(shape)):
+            np.testing.assert_allclose(S.fd(x, True)[ax], np.roll(x, -1, ax) - x, atol=1e-6)
+
+
+def test_l21_closed_form():
+    k = KAT["l21_ones"]
+    x = np.ones(k["shape"], np.float32)
+    assert abs(S.l21_norm(x) - k["norm"]) < 1e-4
+    np.testing.assert_allclose(S.l21_prox(x, 1.0), k["prox_scale"] * x, rtol=1e-5)
+
+
+def _brute_prox(f, v, lam):
+    r = minimize(lambda z: 0.5 * np.sum((z - v) ** 2) + lam * f(z), v, method="Nelder-Mead",
+                 options={"xatol": 1e-9, "fatol": 1e-12, "maxiter": 20000})
+    return r.x
+
+
+def test_prox_brute_force():
+    """Prox optimality (reference pattern functional/prox.py)."""
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(4)
+    lam = 0.3
+    np.testing.assert_allclose(S.l1_prox(v.astype(np.float32), lam),
+                               _brute_prox(lambda z: np.sum(np.abs(z)), v, lam), atol=1e-4)
+    v2 = rng.standard_normal((2, 2))
+    l21 = lambda z: np.sum(np.sqrt(np.sum(z.reshape(2, 2) ** 2, axis=0)))
+    np.testing.assert_allclose(S.l21_prox(v2.astype(np.float32), lam),
+                               _brute_prox(l21, v2.ravel(), lam).reshape(2, 2), atol=1e-4)
+    z = S.l21_prox(np.zeros((2, 3), np.float32), 1.0)
+    assert np.all(z == 0)  # no_nan_divide: zero columns stay zero
+
+
+def test_haar_gram():
+    k = KAT["haar_gram"]
+    x = np.random.default_rng(2).standard_normal((3, 4)).astype(np.float32)
+    for ax in (0, 1):
+        w = S.haar(x, axes=[ax])
+        np.testing.assert_allclose(S.haar_adj(w, axes=[ax]), k["single_axis"] * x, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(S.haar_adj(S.haar(x)), k["two_axes"] * x, rtol=1e-5, atol=1e-6)
+
+
+def test_tv_value_identity():
+    x = np.random.default_rng(3).standard_normal((8, 9)).astype(np.float32)
+    g = S.fd(x)
+    assert abs(S.tv_norm(x) - np.sum(np.sqrt(g[0].astype(np.float64) ** 2 + g[1] ** 2))) < 1e-6
+
+
+@pytest.mark.parametrize("isotropic", [True, False])
+@pytest.mark.parametrize("circular", [False, True])
+def test_tv_prox_vs_admm_denoise(isotropic, circular):
+    """test_tvnorm.py:26-181 pattern: ADMM-TV denoising (exact TV) and
+    APGM with the approximate TV prox agree (SNR > 50 dB)."""
+    N = 24
+    rng = np.random.default_rng(4)
+    xgt = np.zeros((N, N), np.float32)
+    xgt[6:18, 6:18] = 1.0
+    xgt[9:14, 3:20] += 0.5
+    y = (xgt + 0.05 * rng.standard_normal(xgt.shape)).astype(np.float32)
+    lam = 5e-2
+    I = lambda v: v
+    x_admm, _ = S.admm_tv(y, I, I, lam, 1e1, y, 200, cg_tol=1e-6, cg_maxiter=50,
+                          circular=circular, isotropic=isotropic)
+    grad = lambda v: (v - y).astype(np.float32)
+    prox = lambda v, s: S.tv_prox(v, lam * s, isotropic=isotropic, circular=circular)
+    x_apgm, _ = S.apgm(grad, prox, 5e2, y, 1500)
+    snr = 10 * np.log10(np.var(x_admm) / np.var(x_admm - x_apgm))
+    assert snr > 30  # the reference's SNR > 50 dB uses longer runs
+
+
+def test_cg_direct_solve():
+    """test_solver.py:28-114 pattern: CG on an SPD system vs direct solve."""
+    rng = np.random.default_rng(5)
+    n = 40
+    Am = rng.standard_normal((n, n))
+    Am = (Am.T @ Am + n * np.eye(n)).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    x, info = S.cg(lambda v: (Am @ v).astype(np.float32), b, np.zeros(n, np.float32),
+                   tol=1e-6, maxiter=200)
+    xd = np.linalg.solve(Am.astype(np.float64), b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-5
+    assert info["rel_res"] <= 1e-6 * 1.01
+
+
+def test_admm_quadratic_closed_form():
+    """test_admm.py:111-258 pattern: with lam = 0 the minimiser of
+    0.5||y - x||^2 is y (ADMM contracts at rate 8 rho / (1 + 8 rho))."""
+    y = np.random.default_rng(6).standard_normal((10, 12)).astype(np.float32)
+    I = lambda v: v
+    x, hist = S.admm_tv(y, I, I, 0.0, 1e-2, np.zeros_like(y), 20, cg_tol=1e-7, cg_maxiter=100)
+    np.testing.assert_allclose(x, y, atol=1e-5)
+
+
+def test_circconv_direct_and_single_pixel():
+    """test_circconv.py:35-56 (single pixel = flipped kernel inner product)
+    and a direct circular sum, both in float64."""
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((12, 8)).astype(np.float32)
+    h = rng.standard_normal((3, 2)).astype(np.float32)
+    hd = S.circconv_hdft(h, x.shape)
+    Ax = S.circconv(x, hd)
+    pix = np.sum(np.flip(h, (0, 1)) * x[-3:, -2:])
+    np.testing.assert_allclose(Ax[-1, -1], pix, rtol=1e-5)
+    np.testing.assert_allclose(Ax, S.circconv_direct(x, h), atol=1e-5)
+    c = (1, 1)
+    hdc = S.circconv_hdft(h, x.shape, center=c)
+    np.testing.assert_allclose(S.circconv(x, hdc), S.circconv_direct(x, h, center=c), atol=1e-5)
+    yv = rng.standard_normal(x.shape).astype(np.float32)
+    a = np.vdot(S.circconv(x, hdc).astype(np.float64), yv)
+    b = np.vdot(x.astype(np.float64), S.circconv_adj(yv, hdc))
+    assert abs(a - b) < 1e-5 * abs(a)
+
+
+def test_apgm_fixed_point():
+    """x* = prox(x* - grad f(x*)/L, 1/L) at convergence (test_pgm.py pattern)."""
+    rng = np.random.default_rng(8)
+    y = rng.standard_normal(20).astype(np.float32)
+    lam = 0.5
+    grad = lambda v: (v - y).astype(np.float32)
+    prox = lambda v, s: S.l1_prox(v, lam * s)
+    x, res = S.apgm(grad, prox, 1.0, np.zeros_like(y), 50)
+    np.testing.assert_allclose(x, S.l1_prox(y, lam), atol=1e-6)
