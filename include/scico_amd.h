@@ -89,6 +89,12 @@ int scxr_xr2d_adjoint(const scxr_xr2d_plan* plan, const float* sino, float* im,
 int scxr_xr2d_weights(const scxr_xr2d_plan* plan, int32_t* inds, float* weights,
                       void* stream);
 
+/* FBP row filter (scico/linop/xray/_xray2d.py:158-221): out (A, Dout) =
+ * rows of y (A, D) linearly convolved with taps (device, D), keeping
+ * outputs [shift, shift + Dout) of the full convolution. D <= 4096. */
+int scxr_ramp_filter(int32_t A, int32_t D, int32_t Dout, int32_t shift, const float* taps,
+                     const float* y, float* out, const int32_t* pred, void* stream);
+
 /* Launch facts: out[0] forward workgroups (= partial count of
  * scxr_xr2d_forward_res), out[1] adjoint workgroups (= partial count of the
  * adjoint dot epilogue), out[2] views per forward workgroup, out[3] adjoint
@@ -143,6 +149,15 @@ int scxr_vec_scale(int64_t n, float a, const float* x, float* z, const int32_t* 
                    void* stream);
 int scxr_vec_fill(int64_t n, float v, float* z, const int32_t* pred, void* stream);
 int scxr_vec_copy(int64_t n, const float* x, float* z, const int32_t* pred, void* stream);
+/* z = x * y elementwise. */
+int scxr_vec_mul(int64_t n, const float* x, const float* y, float* z, const int32_t* pred,
+                 void* stream);
+/* z = (x >= thr) ? v : 0. */
+int scxr_vec_ge(int64_t n, const float* x, float thr, float v, float* z, const int32_t* pred,
+                void* stream);
+/* z = x / *d with a device float64 divisor (0 divisor leaves x). */
+int scxr_vec_div_dev(int64_t n, const float* x, const double* d, float* z, const int32_t* pred,
+                     void* stream);
 /* *out = post(scale * reduction(x, y)). */
 int scxr_vec_reduce(int32_t kind, int64_t n, int32_t K, const float* x, const float* y,
                     double scale, int32_t post, double* out, double* ws, const int32_t* pred,
@@ -206,6 +221,46 @@ int scxr_admm_itstat(const double* res, int32_t nres, const double* g, int32_t n
                      const double* prml, int32_t nprml, const double* dual, int32_t ndual,
                      const double* cg_state, double lam, double rho, double* hist,
                      int32_t* counter, int32_t hist_cap, void* stream);
+
+/* ----------------------------------------------------------------- TV --
+ * Approximate scaled prox of IsotropicTVNorm / AnisotropicTVNorm over all
+ * axes (scico/functional/_tvnorm.py:117-236: Haar transform of the
+ * edge-padded input, shrink of the high-pass, adjoint, crop, times 1/K),
+ * fused into one stencil pass; tau = sqrt(2) * K * lam, K = 2 * ndim.
+ * out must not alias v.
+ */
+int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out, float tau,
+                 int32_t isotropic, int32_t circular, const int32_t* pred, void* stream);
+/* TV value norm(FD(x)) (scico/functional/_tvnorm.py:89-115) into the device
+ * double *out; ws: float64[scxr_elem_blocks(n) + 1]. */
+int scxr_tv_norm(int32_t ndim, const int32_t* shape, const float* x, int32_t isotropic,
+                 int32_t circular, double* out, double* ws, const int32_t* pred, void* stream);
+
+/* ------------------------------------------------ CircularConvolve / FFT --
+ * Replaces scico.linop.CircularConvolve (scico/linop/_circconv.py:79-209)
+ * for real float32 input of shape (batch..., F0[, F1[, F2]]).  FFT lengths:
+ * powers of two up to 4096 (radix-2 in LDS) or any length up to 512
+ * (direct DFT).  The plan owns the spectrum (complex64, F-shaped) and a
+ * complex workspace of batch * prod(F).
+ */
+typedef struct scxr_cconv_plan scxr_cconv_plan;
+int scxr_cconv_plan_create(int32_t ndim, const int32_t* fshape /* host */, int64_t batch,
+                           scxr_cconv_plan** out);
+int scxr_cconv_plan_destroy(scxr_cconv_plan* plan);
+/* Spectrum = fftn(h) of the real, zero-padded F-shaped h (h_is_dft = 0) or
+ * a copy of the complex64 h (h_is_dft = 1); then times shift (complex64,
+ * nullable) -- the h_center shift of _circconv.py:155-186. */
+int scxr_cconv_set_spectrum(scxr_cconv_plan* plan, const float* h, int32_t h_is_dft,
+                            const float* shift, void* stream);
+/* Copy the complex64 spectrum to out (device). */
+int scxr_cconv_spectrum(const scxr_cconv_plan* plan, float* out, void* stream);
+/* y = real(ifftn(H * fftn(x))) or, adjoint != 0, with conj(H)
+ * (_circconv.py:188-209). */
+int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int32_t adjoint,
+                     const int32_t* pred, void* stream);
+/* Complex fftn / ifftn over the plan's F axes (x real if x_real != 0). */
+int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,
+              int32_t inverse, const int32_t* pred, void* stream);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,7 @@ SOURCES = [
     "xray2d.hip",
     "vec.hip",
     "solver.hip",
+    "fft.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -88,6 +88,23 @@ SIGNATURES = [
     ("scxr_admm_itstat", c_int32,
      [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
       c_double, c_double, c_void_p, c_void_p, c_int32, c_void_p]),
+    # TV
+    ("scxr_tv_prox", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_void_p, c_void_p]),
+    ("scxr_tv_norm", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    # CircularConvolve / FFT
+    ("scxr_cconv_plan_create", c_int32, [c_int32, c_int32_p, c_int64, POINTER(c_void_p)]),
+    ("scxr_cconv_plan_destroy", c_int32, [c_void_p]),
+    ("scxr_cconv_set_spectrum", c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_cconv_spectrum", c_int32, [c_void_p, c_void_p, c_void_p]),
+    ("scxr_cconv_apply", c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_fftn", c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_ramp_filter", c_int32,
+     [c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_vec_mul", c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_vec_ge", c_int32, [c_int64, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p]),
+    ("scxr_vec_div_dev", c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     # vector kernels
     ("scxr_vec_axpby", c_int32,
      [c_int64, c_float, c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p]),

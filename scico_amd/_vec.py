@@ -74,3 +74,26 @@ def dot(x, y, **kw):
 def norm(x, **kw):
     """Euclidean norm (float64 accumulation)."""
     return reduce(RED_SUMSQ, x, post=POST_SQRT, **kw)
+
+
+def mul(x, y, out=None, pred=None):
+    out = torch.empty_like(x) if out is None else out
+    call("scxr_vec_mul", _n(x), x.data_ptr(), y.data_ptr(), out.data_ptr(),
+         dev.ptr(_pred.resolve(pred)), dev.stream())
+    return out
+
+
+def ge(x, thr, v=1.0, out=None, pred=None):
+    """``(x >= thr) * v``."""
+    out = torch.empty_like(x) if out is None else out
+    call("scxr_vec_ge", _n(x), x.data_ptr(), float(thr), float(v), out.data_ptr(),
+         dev.ptr(_pred.resolve(pred)), dev.stream())
+    return out
+
+
+def div_dev(x, d, out=None, pred=None):
+    """``x / d`` with ``d`` a 0-d float64 device tensor."""
+    out = torch.empty_like(x) if out is None else out
+    call("scxr_vec_div_dev", _n(x), x.data_ptr(), d.data_ptr(), out.data_ptr(),
+         dev.ptr(_pred.resolve(pred)), dev.stream())
+    return out

@@ -1,0 +1,416 @@
+// Multi-dimensional complex FFT and DFT-domain circular convolution
+// (scico/linop/_circconv.py:79-209) for gfx950.
+//
+// Layout: data (B, F0, ..., F{d-1}) row-major, B = product of the leading
+// non-transformed axes, d <= 3 transformed axes.  One kernel performs one
+// pass of 1D transforms along one axis; each workgroup loads LPB lines
+// into LDS, transforms them and writes them back:
+//   * power-of-two lengths <= 4096: iterative radix-2 decimation in time
+//     (bit-reversed load, log2(n) butterfly stages);
+//   * other lengths <= 512: direct DFT (out-of-place in LDS).
+// Twiddles are float32 roundings of float64 exp(-2 pi i k / n).
+// Inverse passes scale by 1/n, like numpy / jax ifft.
+//
+// Convolution y = real(ifftn(H * fftn(x))): forward passes on F{d-1} ..
+// F1, one fused pass on F0 (forward, multiply by H or conj(H), inverse),
+// inverse passes on F1 .. F{d-1}; the first pass reads the real input and
+// the last one writes the real part.  With d == 1 the single fused pass
+// does both.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+using namespace scxr;
+
+namespace scxr {
+namespace fft {
+
+constexpr int MAX_POW2 = 4096;
+constexpr int MAX_DFT = 512;
+constexpr int LDS_ELEMS = 4096;  // float2 of LDS per workgroup (32 KiB)
+
+struct Pass {
+  int n, log2n;          // line length, log2 (pow2) or -1 (direct DFT)
+  long long stride;      // element stride along the line
+  long long inner;       // lines per outer index (= stride)
+  long long lines;       // total lines
+  int lpb;               // lines per workgroup
+  int in_real;           // input is real float32
+  int out_real;          // output real part (float32)
+  int inverse;           // direction of the (non-fused) transform
+  int fused;             // forward, multiply by h (conj if conj_h), inverse
+  int conj_h;
+  const float2* tw;      // pow2: n/2 twiddles; DFT: n twiddles
+  const float2* h;       // fused: spectrum, indexed by k * stride + inner
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(__fsub_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)),
+                     __fadd_rn(__fmul_rn(a.x, b.y), __fmul_rn(a.y, b.x)));
+}
+
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// In-place radix-2 DIT stages on lpb lines of length n stored at buf[l*n + k]
+// (input in bit-reversed order, output in natural order).
+__device__ void dit_stages(float2* buf, int n, int log2n, int lpb, const float2* __restrict__ tw,
+                           bool inv) {
+  const int half_total = lpb * (n >> 1);
+  for (int s = 1; s <= log2n; ++s) {
+    const int half = 1 << (s - 1);
+    const int tstep = n >> s;  // twiddle index step
+    __syncthreads();
+    for (int b = threadIdx.x; b < half_total; b += blockDim.x) {
+      const int l = b >> (log2n - 1);
+      const int j = b & ((n >> 1) - 1);
+      const int pos = j & (half - 1);
+      const int i0 = ((j >> (s - 1)) << s) + pos;
+      float2* line = buf + l * n;
+      float2 w = tw[pos * tstep];
+      if (inv) w = conjf2(w);
+      const float2 a = line[i0];
+      const float2 t = cmul(w, line[i0 + half]);
+      line[i0] = make_float2(__fadd_rn(a.x, t.x), __fadd_rn(a.y, t.y));
+      line[i0 + half] = make_float2(__fsub_rn(a.x, t.x), __fsub_rn(a.y, t.y));
+    }
+  }
+  __syncthreads();
+}
+
+// Direct DFT of lpb lines from src to dst (both [l*n + k]).
+__device__ void direct_dft(const float2* src, float2* dst, int n, int lpb,
+                           const float2* __restrict__ tw, bool inv) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < lpb * n; e += blockDim.x) {
+    const int l = e / n, k = e - l * n;
+    const float2* x = src + l * n;
+    float re = 0.0f, im = 0.0f;
+    int idx = 0;
+    for (int m = 0; m < n; ++m) {
+      float2 w = tw[idx];
+      if (inv) w = conjf2(w);
+      const float2 p = cmul(x[m], w);
+      re = __fadd_rn(re, p.x);
+      im = __fadd_rn(im, p.y);
+      idx += k;
+      if (idx >= n) idx -= n;
+    }
+    dst[e] = make_float2(re, im);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int bitrev(int k, int log2n) {
+  return static_cast<int>(__builtin_bitreverse32(static_cast<unsigned>(k)) >> (32 - log2n));
+}
+
+__global__ __launch_bounds__(256) void pass_kernel(const void* __restrict__ in,
+                                                   void* __restrict__ out, Pass P, float scale,
+                                                   const int32_t* pred) {
+  if (pred_off(pred)) return;
+  extern __shared__ float2 lds[];  // two buffers of lpb * n
+  __shared__ long long base[64];
+  const int n = P.n, lpb = P.lpb, tot = lpb * n;
+  float2* A = lds;
+  float2* B = lds + tot;
+  const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
+  const int nl = static_cast<int>(min(static_cast<long long>(lpb), P.lines - line0));
+  if (threadIdx.x < lpb) {
+    const long long L = line0 + threadIdx.x;
+    const long long outer = L / P.inner, inr = L - outer * P.inner;
+    base[threadIdx.x] = outer * n * P.stride + inr;
+  }
+  __syncthreads();
+  const bool pow2 = P.log2n >= 0;
+  const bool contiguous = P.stride == 1;
+  auto split = [&](int e, int& l, int& k) {
+    if (contiguous) {
+      l = e / n;
+      k = e - l * n;
+    } else {
+      k = e / lpb;
+      l = e - k * lpb;
+    }
+  };
+  // load (bit-reversed for the radix-2 path)
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, k;
+    split(e, l, k);
+    float2 v = make_float2(0.0f, 0.0f);
+    if (l < nl) {
+      const long long g = base[l] + static_cast<long long>(k) * P.stride;
+      v = P.in_real ? make_float2(static_cast<const float*>(in)[g], 0.0f)
+                    : static_cast<const float2*>(in)[g];
+    }
+    A[l * n + (pow2 ? bitrev(k, P.log2n) : k)] = v;
+  }
+  // forward (or the only) transform: result in natural order in R
+  const bool inv0 = P.fused ? false : (P.inverse != 0);
+  float2* R;
+  if (pow2) {
+    dit_stages(A, n, P.log2n, lpb, P.tw, inv0);
+    R = A;
+  } else {
+    direct_dft(A, B, n, lpb, P.tw, inv0);
+    R = B;
+  }
+  if (P.fused) {
+    // multiply by the spectrum into the other buffer, then inverse transform
+    float2* Q = (R == A) ? B : A;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int l = e / n, k = e - l * n;
+      float2 pr = make_float2(0.0f, 0.0f);
+      if (l < nl) {
+        const long long inr = (line0 + l) % P.inner;
+        float2 hv = P.h[static_cast<long long>(k) * P.stride + inr];
+        if (P.conj_h) hv = conjf2(hv);
+        pr = cmul(hv, R[e]);
+      }
+      Q[l * n + (pow2 ? bitrev(k, P.log2n) : k)] = pr;
+    }
+    if (pow2) {
+      dit_stages(Q, n, P.log2n, lpb, P.tw, true);
+      R = Q;
+    } else {
+      direct_dft(Q, R, n, lpb, P.tw, true);  // R is free again
+    }
+  }
+  // store
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, k;
+    split(e, l, k);
+    if (l < nl) {
+      const long long g = base[l] + static_cast<long long>(k) * P.stride;
+      const float2 v = R[l * n + k];
+      if (P.out_real)
+        static_cast<float*>(out)[g] = __fmul_rn(v.x, scale);
+      else
+        static_cast<float2*>(out)[g] = make_float2(__fmul_rn(v.x, scale), __fmul_rn(v.y, scale));
+    }
+  }
+}
+
+// Elementwise complex product c = a * b (spectrum centring shift).
+__global__ __launch_bounds__(256) void cmul_kernel(long long n, const float2* __restrict__ a,
+                                                   const float2* __restrict__ b,
+                                                   float2* __restrict__ c) {
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    c[e] = cmul(a[e], b[e]);
+}
+
+// ------------------------------------------------------------------ plan --
+
+struct AxisTables {
+  int n = 0, log2n = -1;
+  float2* tw = nullptr;
+};
+
+struct Plan {
+  int nd = 0;
+  int F[3] = {1, 1, 1};
+  long long batch = 1;
+  long long fsize = 1;  // product of F
+  AxisTables ax[3];
+  float2* h = nullptr;     // spectrum (fsize)
+  float2* work = nullptr;  // complex workspace (batch * fsize)
+};
+
+int log2_exact(int n) {
+  if (n <= 0 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+int make_tables(AxisTables& t, int n) {
+  t.n = n;
+  t.log2n = log2_exact(n);
+  SCXR_REQUIRE(t.log2n >= 0 ? n <= MAX_POW2 : n <= MAX_DFT, SCXR_ERR_UNSUPPORTED,
+               "FFT length %d unsupported (powers of two up to %d, other lengths up to %d)", n,
+               MAX_POW2, MAX_DFT);
+  const int cnt = t.log2n >= 0 ? std::max(1, n / 2) : n;
+  std::vector<float2> tw(cnt);
+  for (int k = 0; k < cnt; ++k) {
+    const double a = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
+    tw[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  if (hipMalloc(&t.tw, sizeof(float2) * cnt) != hipSuccess)
+    return fail(SCXR_ERR_NOMEM, "hipMalloc of FFT twiddles failed");
+  SCXR_HIP(hipMemcpy(t.tw, tw.data(), sizeof(float2) * cnt, hipMemcpyHostToDevice));
+  return SCXR_OK;
+}
+
+void destroy(Plan& p) {
+  for (auto& a : p.ax)
+    if (a.tw) (void)hipFree(a.tw), a.tw = nullptr;
+  if (p.h) (void)hipFree(p.h), p.h = nullptr;
+  if (p.work) (void)hipFree(p.work), p.work = nullptr;
+}
+
+// One pass along FFT axis a.
+int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool out_real,
+             bool inverse, bool fused, bool conj_h, const int32_t* pred, hipStream_t s) {
+  Pass P{};
+  const AxisTables& t = p.ax[a];
+  P.n = t.n;
+  P.log2n = t.log2n;
+  long long stride = 1;
+  for (int b = a + 1; b < p.nd; ++b) stride *= p.F[b];
+  P.stride = stride;
+  P.inner = stride;
+  P.lines = p.batch * (p.fsize / t.n);
+  // Two LDS buffers for the fused and direct-DFT passes, one otherwise;
+  // at most 64 KiB of LDS per workgroup in total.
+  const bool two = fused || t.log2n < 0;
+  const int cap = two ? LDS_ELEMS / 2 : LDS_ELEMS;
+  SCXR_REQUIRE(t.n <= cap, SCXR_ERR_UNSUPPORTED,
+               "FFT length %d too long for a %s pass (max %d)", t.n,
+               fused ? "fused convolution" : "direct DFT", cap);
+  int lpb = std::max(1, std::min(64, cap / t.n));
+  if (t.log2n >= 0) {  // keep lpb a power of two
+    int q = 1;
+    while (q * 2 <= lpb) q *= 2;
+    lpb = q;
+  }
+  P.lpb = lpb;
+  P.in_real = in_real;
+  P.out_real = out_real;
+  P.inverse = inverse;
+  P.fused = fused;
+  P.conj_h = conj_h;
+  P.tw = t.tw;
+  P.h = p.h;
+  const float scale = (inverse || fused) ? 1.0f / static_cast<float>(t.n) : 1.0f;
+  const size_t lds = sizeof(float2) * (two ? 2 : 1) * lpb * t.n;
+  const unsigned grid = static_cast<unsigned>((P.lines + lpb - 1) / lpb);
+  hipLaunchKernelGGL(pass_kernel, dim3(grid), dim3(256), lds, s, in, out, P, scale, pred);
+  SCXR_LAUNCHED("fft pass");
+  return SCXR_OK;
+}
+
+// y = real(ifftn(H' * fftn(x))) with H' = H or conj(H).
+int convolve(const Plan& p, const float* x, float* y, bool adj, const int32_t* pred,
+             hipStream_t s) {
+  if (p.nd == 1) return run_pass(p, 0, x, y, true, true, false, true, adj, pred, s);
+  int rc;
+  // forward along F{d-1} .. F1 (first pass reads the real input)
+  for (int a = p.nd - 1, first = 1; a >= 1; --a, first = 0) {
+    rc = run_pass(p, a, first ? static_cast<const void*>(x) : p.work, p.work, first, false, false,
+                  false, false, pred, s);
+    if (rc) return rc;
+  }
+  rc = run_pass(p, 0, p.work, p.work, false, false, false, true, adj, pred, s);
+  if (rc) return rc;
+  for (int a = 1; a < p.nd; ++a) {
+    const bool last = (a == p.nd - 1);
+    rc = run_pass(p, a, p.work, last ? static_cast<void*>(y) : p.work, false, last, true, false,
+                  false, pred, s);
+    if (rc) return rc;
+  }
+  return SCXR_OK;
+}
+
+// Complex fftn / ifftn of x (complex or real input) into out (complex).
+int fftn(const Plan& p, const void* x, bool x_real, float2* out, bool inverse, const int32_t* pred,
+         hipStream_t s) {
+  for (int a = p.nd - 1, first = 1; a >= 0; --a, first = 0) {
+    const int rc = run_pass(p, a, first ? x : static_cast<const void*>(out), out,
+                            first && x_real, false, inverse, false, false, pred, s);
+    if (rc) return rc;
+  }
+  return SCXR_OK;
+}
+
+}  // namespace fft
+}  // namespace scxr
+
+struct scxr_cconv_plan {
+  scxr::fft::Plan p;
+};
+
+using namespace scxr::fft;
+
+extern "C" {
+
+int scxr_cconv_plan_create(int32_t ndim, const int32_t* fshape, int64_t batch,
+                           scxr_cconv_plan** out) {
+  SCXR_REQUIRE(out && fshape && ndim >= 1 && ndim <= 3 && batch >= 1, SCXR_ERR_ARG,
+               "bad cconv plan arguments");
+  auto* plan = new scxr_cconv_plan();
+  Plan& p = plan->p;
+  p.nd = ndim;
+  p.batch = batch;
+  p.fsize = 1;
+  int rc = SCXR_OK;
+  for (int a = 0; a < ndim && rc == SCXR_OK; ++a) {
+    if (fshape[a] < 1) rc = fail(SCXR_ERR_SHAPE, "non-positive FFT length");
+    if (rc == SCXR_OK) {
+      p.F[a] = fshape[a];
+      p.fsize *= fshape[a];
+      rc = make_tables(p.ax[a], fshape[a]);
+    }
+  }
+  if (rc == SCXR_OK && hipMalloc(&p.h, sizeof(float2) * p.fsize) != hipSuccess)
+    rc = fail(SCXR_ERR_NOMEM, "hipMalloc of the spectrum failed");
+  if (rc == SCXR_OK && hipMalloc(&p.work, sizeof(float2) * p.fsize * p.batch) != hipSuccess)
+    rc = fail(SCXR_ERR_NOMEM, "hipMalloc of the FFT workspace failed");
+  if (rc != SCXR_OK) {
+    destroy(p);
+    delete plan;
+    return rc;
+  }
+  *out = plan;
+  return SCXR_OK;
+}
+
+int scxr_cconv_plan_destroy(scxr_cconv_plan* plan) {
+  if (!plan) return SCXR_OK;
+  destroy(plan->p);
+  delete plan;
+  return SCXR_OK;
+}
+
+int scxr_cconv_set_spectrum(scxr_cconv_plan* plan, const float* h, int32_t h_is_dft,
+                            const float* shift, void* stream) {
+  SCXR_REQUIRE(plan && h, SCXR_ERR_ARG, "null argument to cconv_set_spectrum");
+  Plan& p = plan->p;
+  const hipStream_t s = as_stream(stream);
+  if (h_is_dft) {
+    SCXR_HIP(hipMemcpyAsync(p.h, h, sizeof(float2) * p.fsize, hipMemcpyDeviceToDevice, s));
+  } else {
+    // h: real, zero-padded to the FFT shape (batch 1)
+    Plan q = p;
+    q.batch = 1;
+    const int rc = fftn(q, h, true, p.h, false, nullptr, s);
+    if (rc) return rc;
+  }
+  if (shift) {
+    hipLaunchKernelGGL(cmul_kernel, dim3(256), dim3(256), 0, s, p.fsize, p.h,
+                       reinterpret_cast<const float2*>(shift), p.h);
+    SCXR_LAUNCHED("cconv shift");
+  }
+  return SCXR_OK;
+}
+
+int scxr_cconv_spectrum(const scxr_cconv_plan* plan, float* out, void* stream) {
+  SCXR_REQUIRE(plan && out, SCXR_ERR_ARG, "null argument to cconv_spectrum");
+  SCXR_HIP(hipMemcpyAsync(out, plan->p.h, sizeof(float2) * plan->p.fsize,
+                          hipMemcpyDeviceToDevice, as_stream(stream)));
+  return SCXR_OK;
+}
+
+int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int32_t adjoint,
+                     const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(plan && x && y, SCXR_ERR_ARG, "null argument to cconv_apply");
+  return convolve(plan->p, x, y, adjoint != 0, pred, as_stream(stream));
+}
+
+int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,
+              int32_t inverse, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(plan && x && out, SCXR_ERR_ARG, "null argument to fftn");
+  return fftn(plan->p, x, x_real != 0, reinterpret_cast<float2*>(out), inverse != 0, pred,
+              as_stream(stream));
+}
+
+}  // extern "C"

@@ -346,6 +346,116 @@ __global__ __launch_bounds__(256) void admm_itstat_kernel(
   }
 }
 
+// ---------------------------------------------------------------- TV --
+
+// Approximate TV prox of scico/functional/_tvnorm.py:183-236, all axes of
+// an nd-array (nd <= 3, dims padded to 3 with leading 1s), evaluated per
+// output pixel p:
+//   vp = v edge-padded by one on every differenced axis (not if circular),
+//   L_a(q) = s (vp(q) + vp(q + e_a)),  H_a(q) = s (vp(q + e_a) - vp(q))
+//   (indices circular in the padded extent), s = 1/sqrt(2);
+//   H' = shrink of H over the axes (L21, isotropic) or per axis (L1) with
+//   threshold tau, on the unpadded region only;
+//   out(p) = (1/K) sum_a [ s (L_a(p) + L_a(p - e_a)) + s (H'_a(p - e_a) - H'_a(p)) ].
+struct TvDims {
+  int d[3];    // array extents
+  int P[3];    // padded (or circular) extents
+  int first;   // first differenced axis (3 - nd)
+  int circ;
+};
+
+__device__ __forceinline__ float tv_vp(const float* __restrict__ v, const TvDims& t, int q0, int q1,
+                                       int q2) {
+  // q in padded coordinates [0, P); edge replication maps P-1 -> d-1
+  const int i0 = min(q0, t.d[0] - 1), i1 = min(q1, t.d[1] - 1), i2 = min(q2, t.d[2] - 1);
+  return v[(static_cast<int64_t>(i0) * t.d[1] + i1) * t.d[2] + i2];
+}
+
+// Haar high-pass coefficients of all axes at q (shrunk if inside the
+// region), and the low-pass of axis `la` at q.
+__device__ __forceinline__ void tv_coeffs(const float* __restrict__ v, const TvDims& t,
+                                          const int* q, float tau, int iso, float* H,
+                                          float* Lo) {
+  const float s = 0.70710677f;  // float32(1 / sqrt(2))
+  const float vq = tv_vp(v, t, q[0], q[1], q[2]);
+  bool inside = true;
+  for (int a = t.first; a < 3; ++a) {
+    int qn[3] = {q[0], q[1], q[2]};
+    qn[a] = (q[a] + 1 == t.P[a]) ? 0 : q[a] + 1;
+    const float vn = tv_vp(v, t, qn[0], qn[1], qn[2]);
+    H[a] = __fmul_rn(s, __fsub_rn(vn, vq));
+    Lo[a] = __fmul_rn(s, __fadd_rn(vq, vn));
+    if (!t.circ && q[a] >= t.d[a]) inside = false;
+  }
+  if (!inside) return;
+  if (iso) {
+    float w[3];
+    int K = 0;
+    for (int a = t.first; a < 3; ++a) w[K++] = H[a];
+    l21_shrink(w, K, tau);
+    K = 0;
+    for (int a = t.first; a < 3; ++a) H[a] = w[K++];
+  } else {
+    for (int a = t.first; a < 3; ++a) H[a] = l1_shrink(H[a], tau);
+  }
+}
+
+__global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ v,
+                                                      float* __restrict__ out, TvDims t,
+                                                      float tau, float invK, int iso,
+                                                      const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  const float s = 0.70710677f;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int p[3] = {static_cast<int>(e / (static_cast<int64_t>(t.d[1]) * t.d[2])),
+                      static_cast<int>((e / t.d[2]) % t.d[1]), static_cast<int>(e % t.d[2])};
+    float Hp[3], Lp[3];
+    tv_coeffs(v, t, p, tau, iso, Hp, Lp);
+    float acc = 0.0f;
+    for (int a = t.first; a < 3; ++a) {
+      int q[3] = {p[0], p[1], p[2]};
+      q[a] = (p[a] == 0) ? t.P[a] - 1 : p[a] - 1;
+      float Hq[3], Lq[3];
+      tv_coeffs(v, t, q, tau, iso, Hq, Lq);
+      const float lo = __fmul_rn(s, __fadd_rn(Lp[a], Lq[a]));
+      const float hi = __fmul_rn(s, __fsub_rn(Hq[a], Hp[a]));
+      const float term = __fadd_rn(lo, hi);
+      acc = (a == t.first) ? term : __fadd_rn(acc, term);
+    }
+    out[e] = __fmul_rn(invK, acc);
+  }
+}
+
+// Partials of TV(x) = norm(FD(x)) with FD append=0 or circular
+// (scico/functional/_tvnorm.py:89-115): L21 over the axes (iso) or L1.
+__global__ __launch_bounds__(256) void tv_norm_kernel(const float* __restrict__ x, TvDims t,
+                                                      int iso, double* __restrict__ ws,
+                                                      const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ double sh[4];
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  const int64_t st[3] = {static_cast<int64_t>(t.d[1]) * t.d[2], t.d[2], 1};
+  double acc = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int p[3] = {static_cast<int>(e / st[0]), static_cast<int>((e / t.d[2]) % t.d[1]),
+                      static_cast<int>(e % t.d[2])};
+    const float xc = x[e];
+    double ss = 0.0;
+    for (int a = t.first; a < 3; ++a) {
+      const int kn = (p[a] + 1 < t.d[a]) ? p[a] + 1 : 0;
+      const float g = fd::diff_fwd(xc, x[e + (kn - p[a]) * st[a]], p[a], t.d[a], t.circ != 0);
+      if (iso)
+        ss += static_cast<double>(g) * g;
+      else
+        acc += fabs(static_cast<double>(g));
+    }
+    if (iso) acc += sqrt(ss);
+  }
+  acc = block_sum_f64(acc, sh);
+  if (threadIdx.x == 0) ws[blockIdx.x] = acc;
+}
+
 unsigned elem_grid(int64_t n) {
   const int64_t b = (n + 1023) / 1024;
   return static_cast<unsigned>(b < 1 ? 1 : (b > ELEM_BLOCKS ? ELEM_BLOCKS : b));
@@ -486,5 +596,47 @@ int scxr_admm_itstat(const double* res, int32_t nres, const double* g, int32_t n
 }
 
 int32_t scxr_elem_blocks(int64_t n) { return static_cast<int32_t>(solver::elem_grid(n)); }
+
+static int tv_dims(int32_t ndim, const int32_t* shape, int32_t circular, solver::TvDims& t) {
+  SCXR_REQUIRE(ndim >= 1 && ndim <= 3 && shape, SCXR_ERR_ARG, "bad TV shape");
+  for (int a = 0; a < 3; ++a) t.d[a] = 1;
+  for (int k = 0; k < ndim; ++k) {
+    SCXR_REQUIRE(shape[k] > 0, SCXR_ERR_SHAPE, "empty array");
+    t.d[3 - ndim + k] = shape[k];
+  }
+  t.first = 3 - ndim;
+  t.circ = circular != 0;
+  for (int a = 0; a < 3; ++a) t.P[a] = (a >= t.first && !t.circ) ? t.d[a] + 1 : t.d[a];
+  return SCXR_OK;
+}
+
+int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out, float tau,
+                 int32_t isotropic, int32_t circular, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(v && out && v != out, SCXR_ERR_ARG, "bad tv_prox arguments (out must not alias v)");
+  solver::TvDims t;
+  const int rc = tv_dims(ndim, shape, circular, t);
+  if (rc) return rc;
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
+                     isotropic, pred);
+  SCXR_LAUNCHED("tv prox");
+  return SCXR_OK;
+}
+
+int scxr_tv_norm(int32_t ndim, const int32_t* shape, const float* x, int32_t isotropic,
+                 int32_t circular, double* out, double* ws, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(x && out && ws, SCXR_ERR_ARG, "bad tv_norm arguments");
+  solver::TvDims t;
+  const int rc = tv_dims(ndim, shape, circular, t);
+  if (rc) return rc;
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  const unsigned g = solver::elem_grid(n);
+  hipLaunchKernelGGL(solver::tv_norm_kernel, dim3(g), dim3(256), 0, as_stream(stream), x, t,
+                     isotropic, ws, pred);
+  SCXR_LAUNCHED("tv norm");
+  return vec::sum_partials(static_cast<int>(g), ws, 1.0, vec::POST_NONE, out, pred,
+                           as_stream(stream));
+}
 
 }  // extern "C"

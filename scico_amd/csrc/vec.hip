@@ -48,6 +48,35 @@ __global__ __launch_bounds__(256) void copy_kernel(int64_t n, const float* __res
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) z[e] = x[e];
 }
 
+__global__ __launch_bounds__(256) void mul_kernel(int64_t n, const float* __restrict__ x,
+                                                  const float* __restrict__ y,
+                                                  float* __restrict__ z, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    z[e] = __fmul_rn(x[e], y[e]);
+}
+
+// z = (x >= thr) ? v : 0
+__global__ __launch_bounds__(256) void ge_kernel(int64_t n, const float* __restrict__ x, float thr,
+                                                 float v, float* __restrict__ z,
+                                                 const int32_t* pred) {
+  if (pred_off(pred)) return;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    z[e] = (x[e] >= thr) ? v : 0.0f;
+}
+
+// z = x * (s / *d) with a device float64 divisor (power iteration).
+__global__ __launch_bounds__(256) void scale_div_kernel(int64_t n, const float* __restrict__ x,
+                                                        const double* __restrict__ d,
+                                                        float* __restrict__ z,
+                                                        const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const double dv = *d;
+  const float inv = dv != 0.0 ? static_cast<float>(1.0 / dv) : 0.0f;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
+    z[e] = dv != 0.0 ? static_cast<float>(static_cast<double>(x[e]) / dv) : x[e] * inv;
+}
+
 // Element term of each reduction kind.
 __device__ __forceinline__ double red_term(int kind, const float* x, const float* y, int64_t e,
                                            int64_t n, int K) {
@@ -126,6 +155,14 @@ int reduce(int kind, int64_t n, int K, const float* x, const float* y, double sc
   return SCXR_OK;
 }
 
+int sum_partials(int nparts, const double* parts, double scale, int post, double* out,
+                 const int32_t* pred, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(256), 0, s, nparts, parts, scale, post, out,
+                     pred);
+  SCXR_LAUNCHED("sum partials");
+  return SCXR_OK;
+}
+
 }  // namespace vec
 }  // namespace scxr
 
@@ -176,6 +213,36 @@ int scxr_vec_copy(int64_t n, const float* x, float* z, const int32_t* pred, void
   hipLaunchKernelGGL(vec::copy_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
                      x, z, pred);
   SCXR_LAUNCHED("copy");
+  return SCXR_OK;
+}
+
+int scxr_vec_mul(int64_t n, const float* x, const float* y, float* z, const int32_t* pred,
+                 void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && y && z)), SCXR_ERR_ARG, "bad mul arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::mul_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     x, y, z, pred);
+  SCXR_LAUNCHED("mul");
+  return SCXR_OK;
+}
+
+int scxr_vec_ge(int64_t n, const float* x, float thr, float v, float* z, const int32_t* pred,
+                void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && z)), SCXR_ERR_ARG, "bad ge arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::ge_kernel, dim3(vec::grid_for(n)), dim3(256), 0, as_stream(stream), n,
+                     x, thr, v, z, pred);
+  SCXR_LAUNCHED("ge");
+  return SCXR_OK;
+}
+
+int scxr_vec_div_dev(int64_t n, const float* x, const double* d, float* z, const int32_t* pred,
+                     void* stream) {
+  SCXR_REQUIRE(n >= 0 && (n == 0 || (x && d && z)), SCXR_ERR_ARG, "bad div_dev arguments");
+  if (n == 0) return SCXR_OK;
+  hipLaunchKernelGGL(vec::scale_div_kernel, dim3(vec::grid_for(n)), dim3(256), 0,
+                     as_stream(stream), n, x, d, z, pred);
+  SCXR_LAUNCHED("div_dev");
   return SCXR_OK;
 }
 

@@ -15,6 +15,9 @@ enum { POST_NONE = 0, POST_SQRT = 1 };
 constexpr int RED_BLOCKS = 1024;
 
 unsigned grid_for(int64_t n);
+// out = post(scale * sum(parts[0..nparts))), one workgroup, fixed order.
+int sum_partials(int nparts, const double* parts, double scale, int post, double* out,
+                 const int32_t* pred, hipStream_t s);
 int reduce(int kind, int64_t n, int K, const float* x, const float* y, double scale, int post,
            double* out, double* ws, const int32_t* pred, hipStream_t s);
 

@@ -498,6 +498,32 @@ __global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino
   }
 }
 
+// FBP row filter (_xray2d.py:187-194): the reference convolves each row
+// with the ramp taps h[0..D) by zero-padded FFT of length 2D - 1 and keeps
+// [s, 2D - 1 - s), s = (D - 1) / 2; here the same linear convolution is
+// summed directly: out[a, t] = sum_m h[m] y[a, t + s - m], 0 <= t + s - m < D.
+__global__ __launch_bounds__(256) void ramp_kernel(const float* __restrict__ y,
+                                                   float* __restrict__ out,
+                                                   const float* __restrict__ taps, int D, int Dout,
+                                                   int s, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float row[4096];
+  __shared__ float h[4096];
+  const int a = blockIdx.x;
+  for (int k = threadIdx.x; k < D; k += 256) {
+    row[k] = y[static_cast<size_t>(a) * D + k];
+    h[k] = taps[k];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < Dout; t += 256) {
+    const int c = t + s;
+    const int m0 = max(0, c - D + 1), m1 = min(D - 1, c);
+    float acc = 0.0f;
+    for (int m = m0; m <= m1; ++m) acc = fmaf(h[m], row[c - m], acc);
+    out[static_cast<size_t>(a) * Dout + t] = acc;
+  }
+}
+
 // Debug dump of (ind, weight) for every (view, pixel).
 __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds,
                                                       float* __restrict__ wts,
@@ -644,6 +670,16 @@ int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out) {
   out[1] = static_cast<int32_t>(ceil_div(p.N, ADJ_TJ) * ceil_div(p.M, ADJ_TI));
   out[2] = p.vpg;
   out[3] = p.adj_w;
+  return SCXR_OK;
+}
+
+int scxr_ramp_filter(int32_t A, int32_t D, int32_t Dout, int32_t shift, const float* taps,
+                     const float* y, float* out, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(A > 0 && D > 0 && D <= 4096 && Dout > 0 && taps && y && out, SCXR_ERR_ARG,
+               "bad ramp_filter arguments (D must be in [1, 4096])");
+  hipLaunchKernelGGL(ramp_kernel, dim3(A), dim3(256), 0, as_stream(stream), y, out, taps, D, Dout,
+                     shift, pred);
+  SCXR_LAUNCHED("ramp filter");
   return SCXR_OK;
 }
 

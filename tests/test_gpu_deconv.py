@@ -1,0 +1,148 @@
+"""GPU parity of the FFT, CircularConvolve, TV prox / norm and APGM-TV
+deconvolution against the oracle (numpy.fft and oracle.solvers)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import solvers as S
+from scico_amd import functional, linop, loss
+from scico_amd.linop import CircularConvolve, valid_adjoint
+from scico_amd.optimize import AcceleratedPGM
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def gaussian_psf(n=15, sigma=2.0):
+    r = np.arange(n) - n // 2
+    g = np.exp(-(r[:, None] ** 2 + r[None, :] ** 2) / (2 * sigma**2))
+    return (g / g.sum()).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,ndims", [((16,), 1), ((12, 8), 2), ((6, 8, 12), 3),
+                                         ((2, 12, 8), 2), ((1024, 1024), 2), ((64, 48), 2),
+                                         ((4, 256), 1)])
+def test_fftn_vs_numpy(gpu, shape, ndims):
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    A = CircularConvolve(np.ones((1,) * ndims, np.float32), shape, ndims=ndims)
+    axes = tuple(range(len(shape) - ndims, len(shape)))
+    for inverse in (False, True):
+        got = A.fftn(dev(x), inverse=inverse).cpu().numpy()
+        f = np.fft.ifftn if inverse else np.fft.fftn
+        ref64 = f(x.astype(np.complex128), axes=axes)
+        e = rel_l2(got, ref64)
+        print(f"  fftn {shape} inv={inverse}: rel-L2 vs float64 {e:.2e}", flush=True)
+        assert e <= 1e-5
+
+
+@pytest.mark.parametrize("x_shape,ndims,h_shape", [((12,), None, (3,)), ((12, 8), None, (3, 2)),
+                                                   ((6, 8, 12), None, (3, 2, 4)),
+                                                   ((2, 12, 8), 2, (3, 2))])
+def test_circconv_reference_patterns(gpu, x_shape, ndims, h_shape):
+    """test_circconv.py:35-70: one output pixel equals the inner product with
+    the flipped kernel; adjoint dot test; direct circular sum."""
+    rng = np.random.default_rng(1)
+    h = rng.standard_normal(h_shape).astype(np.float32)
+    x = rng.standard_normal(x_shape).astype(np.float32)
+    A = CircularConvolve(h, x_shape, ndims)
+    Ax = A(dev(x)).cpu().numpy()
+    nd = A.ndims
+    hf = np.flip(h, tuple(range(-nd, 0)))
+    xi = (...,) + tuple(slice(-h.shape[a], None) for a in range(-nd, 0))
+    Ai = (...,) + tuple(-1 for _ in range(nd))
+    np.testing.assert_allclose(np.sum(hf * x[xi], axis=tuple(-(a + 1) for a in range(nd))),
+                               Ax[Ai], rtol=1e-5, atol=1e-6)
+    assert valid_adjoint(A, A.T, eps=1e-5)
+    if len(x_shape) == nd:
+        np.testing.assert_allclose(Ax, S.circconv_direct(x, h), atol=1e-5)
+
+
+def test_circconv_center_and_from_operator(gpu):
+    """test_circconv.py:151-192: h_center <=> roll; from_operator."""
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, 16).astype(np.float32)
+    h = np.array([0.5, 1.0, 0.25], np.float32)
+    A = CircularConvolve(h, x.shape, h_center=1)
+    B = CircularConvolve(h, x.shape)
+    np.testing.assert_allclose(A(dev(x)).cpu().numpy(), np.roll(B(dev(x)).cpu().numpy(), -1),
+                               atol=1e-5)
+    x2 = rng.standard_normal((12, 8)).astype(np.float32)
+    h2 = rng.standard_normal((3, 2)).astype(np.float32)
+    A2 = CircularConvolve(h2, (12, 8))
+    B2 = CircularConvolve.from_operator(A2)
+    np.testing.assert_allclose(A2(dev(x2)).cpu().numpy(), B2(dev(x2)).cpu().numpy(), atol=1e-5)
+    with pytest.raises(ValueError):
+        CircularConvolve(h2, (12, 8), h_is_dft=True, h_center=[1, 1])
+
+
+def test_circconv_1024_psf(gpu):
+    """1024^2 image, 15x15 Gaussian PSF (sigma 2) centred at (7, 7):
+    forward / adjoint against the oracle (complex64 FFTs) and float64."""
+    rng = np.random.default_rng(3)
+    x = rng.random((1024, 1024)).astype(np.float32)
+    h = gaussian_psf()
+    A = CircularConvolve(h, x.shape, h_center=[7, 7])
+    hd = S.circconv_hdft(h, x.shape, center=[7, 7])
+    y = A(dev(x)).cpu().numpy()
+    yt = A.T(dev(x)).cpu().numpy()
+    e, et = rel_l2(y, S.circconv(x, hd)), rel_l2(yt, S.circconv_adj(x, hd))
+    e64 = rel_l2(y, S.circconv_f64(x, h, center=[7, 7]))
+    print(f"  1024^2 PSF conv: fwd {e:.2e} adj {et:.2e} (vs float64 {e64:.2e})", flush=True)
+    assert e <= 1e-5 and et <= 1e-5
+    np.testing.assert_allclose(A.h_dft.cpu().numpy(), hd, rtol=0, atol=2e-6)
+
+
+@pytest.mark.parametrize("shape", [(33, 31), (8, 9, 10), (40,)])
+@pytest.mark.parametrize("iso", [True, False])
+@pytest.mark.parametrize("circular", [False, True])
+def test_tv_prox_and_norm(gpu, shape, iso, circular):
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(shape).astype(np.float32)
+    F = functional.IsotropicTVNorm(circular=circular) if iso else \
+        functional.AnisotropicTVNorm(circular=circular)
+    lam = 0.13
+    got = (0.7 * F).prox(dev(v), lam).cpu().numpy()
+    ref = S.tv_prox(v, 0.7 * lam, isotropic=iso, circular=circular)
+    assert rel_l2(got, ref) <= 1e-6
+    val = float(F(dev(v)))
+    assert abs(val - S.tv_norm(v, isotropic=iso, circular=circular)) <= 1e-9 * abs(val)
+
+
+def _deconv_problem(N, seed=0):
+    from scico_amd import phantom
+
+    xgt = phantom.ellipse_phantom(N, 12, seed=seed)
+    h = gaussian_psf()
+    hd = S.circconv_hdft(h, (N, N), center=[7, 7])
+    y = S.circconv(xgt, hd)
+    y = phantom.noisy(y, 0.01, seed=seed + 1)
+    L0 = float(np.max(np.abs(hd.astype(np.complex128)) ** 2))
+    return xgt, h, hd, y, L0
+
+
+def test_apgm_tv_deconvolution_parity(gpu):
+    """AcceleratedPGM with isotropic TV (test_tvnorm.py pattern) against the
+    oracle, 20 iterations at 128^2."""
+    N = 128
+    xgt, h, hd, y, L0 = _deconv_problem(N)
+    lam = 2e-2
+    A = CircularConvolve(h, (N, N), h_center=[7, 7])
+    f = loss.SquaredL2Loss(y=dev(y), A=A)
+    g = lam * functional.IsotropicTVNorm()
+    solver = AcceleratedPGM(f=f, g=g, L0=L0, x0=dev(np.zeros((N, N), np.float32)), maxiter=20)
+    x = solver.solve().cpu().numpy()
+    grad = lambda v: S.circconv_adj((S.circconv(v, hd) - y).astype(np.float32), hd)
+    prox = lambda v, s: S.tv_prox(v, lam * s)
+    xr, res = S.apgm(grad, prox, L0, np.zeros((N, N), np.float32), 20)
+    e = rel_l2(x, xr)
+    hist = solver.history(transpose=True)
+    print(f"  APGM-TV 128^2 x20: rel-L2 vs oracle {e:.2e}", flush=True)
+    assert e <= 1e-5
+    np.testing.assert_allclose(hist.Residual, res, rtol=1e-4)
+    assert all(np.isfinite(hist.Objective))

@@ -125,3 +125,40 @@ def test_shape_errors(gpu):
         H.adj(np.zeros((4, 5), np.float32))
     with pytest.raises(ValueError):
         H.adj(np.zeros(H.output_shape, np.float64))
+
+
+@pytest.mark.parametrize("N,nang,dc,dx", [(64, 90, 64, None), (64, 45, 91, None),
+                                          (256, 360, 256, 0.5),
+                                          (256, 360, int(1.02 / np.sqrt(2.0) * 256), 1 / np.sqrt(2))])
+def test_fbp(gpu, N, nang, dc, dx):
+    """fbp vs the oracle (FFT row filter) and the reference's PSNR > 28 dB
+    pin for a 256^2 square with 360 views (test_xray_2d.py:88-102)."""
+    x = np.zeros((N, N), np.float32)
+    x[N // 4 : -N // 4, N // 4 : -N // 4] = 1.0
+    ang = np.linspace(0, np.pi, nang, endpoint=False)
+    kw = {} if dx is None else {"dx": dx}
+    H = XRayTransform2D((N, N), ang, det_count=dc, **kw)
+    y = H(x)
+    xr = H.fbp(y).cpu().numpy()
+    g = X.xr2d_defaults((N, N), ang, det_count=dc, dx=dx)
+    ref = X.fbp(y.cpu().numpy(), g["x0"], g["dx"], g["nx"], g["y0"], g["angles"])
+    e = rel_l2(xr, ref)
+    mse = np.mean((xr.astype(np.float64) - x) ** 2)
+    psnr = 10 * np.log10(1.0 / mse)
+    print(f"  fbp {N}^2/{nang}/{dc}: rel-L2 vs oracle {e:.2e}, PSNR {psnr:.2f} dB", flush=True)
+    assert e <= 1e-5
+    if N == 256:
+        assert psnr > 28
+
+
+def test_operator_norm(gpu):
+    """operator_norm (linop/_util.py:75-110) vs the dense float64 matrix."""
+    from scico_amd.linop import operator_norm
+
+    ang = np.linspace(0, np.pi, 10, endpoint=False)
+    H = XRayTransform2D((12, 13), ang)
+    g = X.xr2d_defaults((12, 13), ang)
+    Hd = X.dense_matrix_overlap(g["x0"], g["dx"], g["nx"], g["angles"], g["y0"], g["ny"])
+    ref = np.linalg.norm(Hd, 2)
+    val = float(operator_norm(H, maxiter=200))
+    assert abs(val - ref) <= 1e-4 * ref
