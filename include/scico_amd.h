@@ -262,6 +262,43 @@ int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int3
 int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,
               int32_t inverse, const int32_t* pred, void* stream);
 
+/* ------------------------------------------------ 3D parallel-beam X-ray --
+ * Replaces scico.linop.xray.XRayTransform3D:
+ *   constructor            scico/linop/xray/_xray3d.py:54-93
+ *   project / _project     scico/linop/xray/_xray3d.py:95-159
+ *   back_project           scico/linop/xray/_xray3d.py:102-108, 161-204
+ *   _calc_weights          scico/linop/xray/_xray3d.py:206-266
+ * Volume (X, Y, Z) row-major, projections (V, D0, D1) row-major; matrices
+ * (host, V x 2 x 4 float32) as the operator holds them (_xray3d.py:80).
+ * Geometries where one detector axis follows one volume axis exactly
+ * (single rotation axis, e.g. matrices_from_euler_angles with seq "Y")
+ * run in slice mode: slice-parallel kernels, deterministic, no atomics.
+ * Others run in general mode (forward with float atomics).  allow_slice = 0
+ * forces general mode.
+ */
+typedef struct scxr_xr3d_plan scxr_xr3d_plan;
+
+int scxr_xr3d_plan_create(int32_t X, int32_t Y, int32_t Z, int32_t V,
+                          const float* matrices /* host, V x 2 x 4 */, int32_t D0,
+                          int32_t D1, int32_t allow_slice, scxr_xr3d_plan** out);
+int scxr_xr3d_plan_destroy(scxr_xr3d_plan* plan);
+/* out (host, 8): mode (0 general, 1 slice), slice volume axis, slice
+ * detector axis, column offset, forward candidates per row, volume used in
+ * place, projections used in place, in-plane detector size. */
+int scxr_xr3d_info(const scxr_xr3d_plan* plan, int32_t* out);
+/* proj = H vol  (_xray3d.py:110-159). */
+int scxr_xr3d_forward(const scxr_xr3d_plan* plan, const float* vol, float* proj,
+                      const int32_t* pred, void* stream);
+/* vol = H^T proj  (_xray3d.py:161-204). */
+int scxr_xr3d_adjoint(const scxr_xr3d_plan* plan, const float* proj, float* vol,
+                      const int32_t* pred, void* stream);
+/* Debug: the reference's weight arrays of one view (_xray3d.py:206-266):
+ * ul (2, X, Y, Z) int32 and w (4, X, Y, Z) float32, from the same device
+ * functions the projection kernels use. */
+int scxr_xr3d_weights(const scxr_xr3d_plan* plan, int32_t view, int32_t* ul, float* w,
+                      void* stream);
+
+
 #ifdef __cplusplus
 }
 #endif

@@ -283,7 +283,7 @@ def matrices_from_euler_angles(input_shape, output_shape, seq, angles, degrees=F
         det_spacing = np.ones(2)
     m = Rotation.from_euler(seq, angles, degrees=degrees).as_matrix()[:, :2, :]
     m = m * np.asarray(voxel_spacing, dtype=np.float64).reshape(1, 1, 3)
-    m = m / np.asarray(det_spacing, dtype=np.float64).reshape(1, 2, 1)
+    m = (1 / np.asarray(det_spacing, dtype=np.float64)).reshape(1, 2, 1) * m
     t = -np.einsum("vmn,n->vm", m, np.array(input_shape) / 2) + np.array(output_shape) / 2
     return np.concatenate((m, t[..., None]), axis=2)
 

@@ -23,6 +23,7 @@ LIB = PKG / "libscico_amd.so"
 SOURCES = [
     "common.cpp",
     "xray2d.hip",
+    "xray3d.hip",
     "vec.hip",
     "solver.hip",
     "fft.hip",

@@ -62,6 +62,15 @@ SIGNATURES = [
     ("scxr_xr2d_adjoint_epi", c_int32,
      [c_void_p, c_void_p, c_void_p, c_int32, c_float, c_int32, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p]),
+    # 3D projector
+    ("scxr_xr3d_plan_create", c_int32,
+     [c_int32, c_int32, c_int32, c_int32, c_float_p, c_int32, c_int32, c_int32,
+      POINTER(c_void_p)]),
+    ("scxr_xr3d_plan_destroy", c_int32, [c_void_p]),
+    ("scxr_xr3d_info", c_int32, [c_void_p, c_int32_p]),
+    ("scxr_xr3d_forward", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_xr3d_adjoint", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_xr3d_weights", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     # finite differences, proxes
     ("scxr_fd_forward", c_int32,
      [c_int32, c_int32_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),

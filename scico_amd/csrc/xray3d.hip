@@ -1,0 +1,745 @@
+// Parallel-beam 3D X-ray projector (scico.linop.xray.XRayTransform3D) for
+// gfx950.
+//
+// Model (scico/linop/xray/_xray3d.py:206-266): voxel (i, j, k) has centre
+// x = (i + 1/2, j + 1/2, k + 1/2); for view v with float32 matrix m (2 x 4)
+//   Px_r = ((m[r][0] x0 + m[r][1] x1) + m[r][2] x2) + m[r][3]
+//   left = Px - 1/4,  t = min(ceil(left) - left, 1/2),  ul = floor(left)
+// and the voxel splats onto the 2 x 2 detector pixels at ul with weights
+// (t0 t1) 4, ((1/2 - t0) t1) 4, (t0 (1/2 - t1)) 4, ((1/2 - t0)(1/2 - t1)) 4,
+// each zeroed when its pixel is off the detector.
+//
+// Slice mode.  When one detector axis d depends only on one volume axis s
+// (the rotation axis) through the same coefficient and offset for every
+// view, and that axis maps slice n exactly onto detector column n + delta
+// with t_d = 1/2, the other two weights of every voxel are exactly zero and
+// the transform splits into independent 2D slices that share one in-plane
+// weight table.  The kernels then run with the SLICE index across the
+// lanes: a wave owns 64 lanes x 4 consecutive slices, computes each
+// (pixel, view) weight once in one lane, broadcasts it with readlane and
+// applies it to 256 slices with float4 loads of contiguous slice rows.
+// Volume and projections are used in a canonical layout with the slice
+// index fastest: vol_c[p][q][s] and proj_c[v][u][s] (p < q the in-plane
+// volume axes, u the in-plane detector axis); a tiled transpose converts
+// to and from the caller's layout when it differs.
+//
+// General mode (any other geometry): the adjoint gathers per voxel in the
+// reference's per-view order; the forward scatters with float atomics.
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace scxr {
+namespace xr3d {
+
+enum { MODE_GENERAL = 0, MODE_SLICE = 1 };
+
+constexpr int FB = 16;  // forward: detector bins per wave
+constexpr int AR = 8;   // adjoint: pixels per wave (2 x 4 block)
+
+struct Plan {
+  int X = 0, Y = 0, Z = 0, V = 0, D0 = 0, D1 = 0;
+  int mode = MODE_GENERAL;
+  int sax = -1, dax = -1, delta = 0, pax = -1, qax = -1;
+  int P = 0, Q = 0, S = 0, SQ = 0, U = 0;  // SQ = slice quads (slices padded to 4)
+  int cpr = 32;                            // forward candidates per row
+  bool vol_direct = false, proj_direct = false;
+  float* mats = nullptr;   // (V, 2, 4) float32
+  float4* geom = nullptr;  // slice mode: (a, b, t, 0) in-plane row per view
+  float* volc = nullptr;   // canonical volume workspace (P Q 4 SQ)
+  float* proja = nullptr;  // canonical projection workspace (V U 4 SQ)
+};
+
+// ------------------------------------------------------------ weights --
+
+struct Tap {
+  int ul;
+  float wl, wr;
+};
+
+// In-plane weights of pixel (p, q) for a slice-mode view with in-plane row
+// (a, b, t): P = (a (p + 1/2) + b (q + 1/2)) + t, the two taps at ul and
+// ul + 1 carry (t 1/2) 4 and ((1/2 - t) 1/2) 4 (the slice axis has
+// t_d = 1/2), zeroed off the detector.
+__device__ __forceinline__ Tap slice_tap(float a, float b, float t, int p, int q, int U) {
+  const float xp = __fadd_rn(static_cast<float>(p), 0.5f);
+  const float xq = __fadd_rn(static_cast<float>(q), 0.5f);
+  const float P = __fadd_rn(__fadd_rn(__fmul_rn(a, xp), __fmul_rn(b, xq)), t);
+  const float left = __fsub_rn(P, 0.25f);
+  const float tt = fminf(__fsub_rn(ceilf(left), left), 0.5f);
+  Tap r;
+  r.ul = static_cast<int>(floorf(left));
+  r.wl = __fmul_rn(__fmul_rn(tt, 0.5f), 4.0f);
+  r.wr = __fmul_rn(__fmul_rn(__fsub_rn(0.5f, tt), 0.5f), 4.0f);
+  if (!(r.ul >= 0 && r.ul < U)) r.wl = 0.0f;
+  if (!(r.ul + 1 >= 0 && r.ul + 1 < U)) r.wr = 0.0f;
+  return r;
+}
+
+// Full reference weights of voxel (i, j, k) for matrix m (general mode).
+__device__ __forceinline__ void gen_weights(const float* __restrict__ m, int i, int j, int k,
+                                            int D0, int D1, int ul[2], float w[4]) {
+  const float x0 = __fadd_rn(static_cast<float>(i), 0.5f);
+  const float x1 = __fadd_rn(static_cast<float>(j), 0.5f);
+  const float x2 = __fadd_rn(static_cast<float>(k), 0.5f);
+  float t[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const float* mr = m + 4 * r;
+    const float P = __fadd_rn(
+        __fadd_rn(__fadd_rn(__fmul_rn(mr[0], x0), __fmul_rn(mr[1], x1)), __fmul_rn(mr[2], x2)),
+        mr[3]);
+    const float left = __fsub_rn(P, 0.25f);
+    t[r] = fminf(__fsub_rn(ceilf(left), left), 0.5f);
+    ul[r] = static_cast<int>(floorf(left));
+  }
+  const float s0 = __fsub_rn(0.5f, t[0]), s1 = __fsub_rn(0.5f, t[1]);
+  w[0] = __fmul_rn(__fmul_rn(t[0], t[1]), 4.0f);
+  w[1] = __fmul_rn(__fmul_rn(s0, t[1]), 4.0f);
+  w[2] = __fmul_rn(__fmul_rn(t[0], s1), 4.0f);
+  w[3] = __fmul_rn(__fmul_rn(s0, s1), 4.0f);
+  const bool a0 = ul[0] >= 0 && ul[0] < D0, b0 = ul[0] + 1 >= 0 && ul[0] + 1 < D0;
+  const bool a1 = ul[1] >= 0 && ul[1] < D1, b1 = ul[1] + 1 >= 0 && ul[1] + 1 < D1;
+  if (!(a0 && a1)) w[0] = 0.0f;
+  if (!(b0 && a1)) w[1] = 0.0f;
+  if (!(a0 && b1)) w[2] = 0.0f;
+  if (!(b0 && b1)) w[3] = 0.0f;
+}
+
+__device__ __forceinline__ float4 fma4(float w, float4 x, float4 acc) {
+  return make_float4(__builtin_fmaf(w, x.x, acc.x), __builtin_fmaf(w, x.y, acc.y),
+                     __builtin_fmaf(w, x.z, acc.z), __builtin_fmaf(w, x.w, acc.w));
+}
+
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float rlf(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// ------------------------------------------------------ slice kernels --
+
+// Forward: each wave owns FB consecutive bins of one view for 256 slices
+// and gathers the strip of pixels that touch them, row by row along the
+// slow in-plane axis.  Lane kk of a row pass holds candidate kk of the
+// row, ordered so that its bin is non-decreasing; the wave walks the
+// candidates group by group (bin ul = u0 - 1 .. u0 + FB - 1), so every
+// accumulator index is static.
+template <int CPR>
+__global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict__ vol,
+                                                        float4* __restrict__ proj,
+                                                        const float4* __restrict__ geom, int P,
+                                                        int Q, int SQ, int U, int V,
+                                                        const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  constexpr int RPP = 64 / CPR;
+  constexpr int BIG = INT_MAX / 2;
+  const int lane = threadIdx.x & 63;
+  const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (v >= V) return;
+  const int u0 = blockIdx.x * FB;
+  const int quad = blockIdx.z * 64 + lane;
+  const bool qok = quad < SQ;
+  const int qld = qok ? quad : 0;
+  const float4 g = geom[v];
+  const float a = g.x, b = g.y, t = g.z;
+  const bool qfast = fabsf(b) >= fabsf(a);
+  const float f = qfast ? b : a, sc = qfast ? a : b;
+  const int nslow = qfast ? P : Q, nfast = qfast ? Q : P;
+  const float plo = static_cast<float>(u0) - 0.75f;
+  const float phi = static_cast<float>(u0 + FB) + 0.25f;
+  const float rf = 1.0f / f;
+  const int kk = lane % CPR;
+
+  float4 acc[FB];
+#pragma unroll
+  for (int j = 0; j < FB; ++j) acc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  for (int r0 = 0; r0 < nslow; r0 += RPP) {
+    const int row = r0 + lane / CPR;
+    // Approximate fast-index interval of the strip in this row, widened by
+    // one pixel each side; the exact bin of each candidate decides.
+    const float c = sc * (static_cast<float>(row) + 0.5f) + t;
+    const float ka = (plo - c) * rf - 0.5f, kb = (phi - c) * rf - 0.5f;
+    const int klo = max(0, static_cast<int>(floorf(fminf(ka, kb))) - 1);
+    const int khi = min(nfast - 1, static_cast<int>(ceilf(fmaxf(ka, kb))) + 1);
+    int gi = BIG, pix = 0;
+    float wl = 0.0f, wr = 0.0f;
+    if (row < nslow && kk <= khi - klo) {
+      const int k = (f > 0.0f) ? klo + kk : khi - kk;
+      const int p = qfast ? row : k, q = qfast ? k : row;
+      const Tap tp = slice_tap(a, b, t, p, q, U);
+      gi = tp.ul - u0;
+      wl = tp.wl;
+      wr = tp.wr;
+      pix = p * Q + q;
+    }
+#pragma unroll
+    for (int rr = 0; rr < RPP; ++rr) {
+      int idx = rr * CPR;
+      const int end = idx + CPR;
+      while (idx < end && rl(gi, idx) < -1) ++idx;
+#pragma unroll
+      for (int gg = -1; gg < FB; ++gg) {
+        while (idx < end && rl(gi, idx) == gg) {
+          const float cl = rlf(wl, idx), cr = rlf(wr, idx);
+          const size_t o = static_cast<size_t>(rl(pix, idx)) * SQ;
+          const float4 x = vol[o + qld];
+          if (gg >= 0) acc[gg] = fma4(cl, x, acc[gg]);
+          if (gg + 1 < FB) acc[gg + 1] = fma4(cr, x, acc[gg + 1]);
+          ++idx;
+        }
+      }
+    }
+  }
+  if (!qok) return;
+#pragma unroll
+  for (int j = 0; j < FB; ++j)
+    if (u0 + j < U) proj[(static_cast<size_t>(v) * U + u0 + j) * SQ + quad] = acc[j];
+}
+
+// Adjoint: each wave owns a 2 x 4 block of pixels for 256 slices and walks
+// the views in order, 8 at a time: lane (view j, pixel r) computes that
+// pair's taps, then every (view, pixel) pair is broadcast and applied as
+// acc += wl * proj_c[v][ul] + wr * proj_c[v][ul + 1] (float4 per lane).
+__global__ __launch_bounds__(256) void adj_slice_kernel(const float4* __restrict__ proj,
+                                                        float4* __restrict__ vol,
+                                                        const float4* __restrict__ geom, int P,
+                                                        int Q, int SQ, int U, int V,
+                                                        const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p0 = blockIdx.y * 4 + 2 * (w >> 1), q0 = blockIdx.x * 8 + 4 * (w & 1);
+  const int quad = blockIdx.z * 64 + lane;
+  const bool qok = quad < SQ;
+  const int qld = qok ? quad : 0;
+  const int r = lane & (AR - 1), jv = lane / AR;
+  const int pp = p0 + (r >> 2), qq = q0 + (r & 3);
+  const bool pok = pp < P && qq < Q;
+
+  float4 acc[AR];
+#pragma unroll
+  for (int i = 0; i < AR; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  for (int v0 = 0; v0 < V; v0 += 64 / AR) {
+    const int vv = v0 + jv;
+    int ul = 0;
+    float wl = 0.0f, wr = 0.0f;
+    if (vv < V && pok) {
+      const float4 g = geom[vv];
+      const Tap tp = slice_tap(g.x, g.y, g.z, pp, qq, U);
+      ul = tp.ul;
+      wl = tp.wl;
+      wr = tp.wr;
+    }
+    const int nv = min(64 / AR, V - v0);
+    for (int j = 0; j < nv; ++j) {
+      const size_t vb = static_cast<size_t>(v0 + j) * U;
+#pragma unroll
+      for (int i = 0; i < AR; ++i) {
+        const int L = j * AR + i;
+        const int u = rl(ul, L);
+        const float cl = rlf(wl, L), cr = rlf(wr, L);
+        const int lo = min(max(u, 0), U - 1), hi = min(max(u + 1, 0), U - 1);
+        const float4 ylo = proj[(vb + lo) * SQ + qld];
+        const float4 yhi = proj[(vb + hi) * SQ + qld];
+        acc[i] = fma4(cr, yhi, fma4(cl, ylo, acc[i]));
+      }
+    }
+  }
+  if (!qok) return;
+#pragma unroll
+  for (int i = 0; i < AR; ++i) {
+    const int p = p0 + (i >> 2), q = q0 + (i & 3);
+    if (p < P && q < Q) vol[(static_cast<size_t>(p) * Q + q) * SQ + quad] = acc[i];
+  }
+}
+
+// ----------------------------------------------------- layout kernels --
+
+// Batched 2D transpose with offsets and zero fill:
+//   out[b ob + c oc + r] = in[b ib + (r + roff) ir + (c + coff)]
+// when 0 <= r + roff < Rin and 0 <= c + coff < Cin, else 0, for
+// r < Rout, c < Cout.  32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, int Rout,
+                                                        int Cout, int Rin, int Cin, int roff,
+                                                        int coff, long long ib, long long ir,
+                                                        long long ob, long long oc,
+                                                        const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  __shared__ float tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const long long b = blockIdx.z;
+  for (int rr = ty; rr < 32; rr += 8) {
+    const int ri = r0 + rr + roff, ci = c0 + tx + coff;
+    float val = 0.0f;
+    if (ri >= 0 && ri < Rin && ci >= 0 && ci < Cin) val = in[b * ib + ri * ir + ci];
+    tile[rr][tx] = val;
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 32; cc += 8) {
+    const int c = c0 + cc, r = r0 + tx;
+    if (c < Cout && r < Rout) out[b * ob + c * oc + r] = tile[tx][cc];
+  }
+}
+
+// Row copy with a column shift and zero fill:
+//   out[row os + s] = in[row is + s + off] if 0 <= s + off < Cin else 0.
+__global__ __launch_bounds__(256) void shift_rows_kernel(const float* __restrict__ in,
+                                                         float* __restrict__ out, long long rows,
+                                                         int Cout, int Cin, int off, long long is,
+                                                         long long os,
+                                                         const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  const long long n = rows * Cout;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += 256LL * gridDim.x) {
+    const long long row = e / Cout;
+    const int s = static_cast<int>(e - row * Cout);
+    const int c = s + off;
+    out[row * os + s] = (c >= 0 && c < Cin) ? in[row * is + c] : 0.0f;
+  }
+}
+
+// ------------------------------------------------- general-mode kernels --
+
+__global__ __launch_bounds__(256) void gen_adj_kernel(const float* __restrict__ proj,
+                                                      float* __restrict__ vol,
+                                                      const float* __restrict__ mats, int X,
+                                                      int Y, int Z, int V, int D0, int D1,
+                                                      const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  const long long n = static_cast<long long>(X) * Y * Z;
+  const long long e = blockIdx.x * 256LL + threadIdx.x;
+  if (e >= n) return;
+  const int k = static_cast<int>(e % Z), j = static_cast<int>((e / Z) % Y),
+            i = static_cast<int>(e / (static_cast<long long>(Y) * Z));
+  float acc = 0.0f;
+  for (int v = 0; v < V; ++v) {
+    int ul[2];
+    float w[4];
+    gen_weights(mats + 8 * v, i, j, k, D0, D1, ul, w);
+    const float* y = proj + static_cast<size_t>(v) * D0 * D1;
+    const int a0 = min(max(ul[0], 0), D0 - 1), b0 = min(max(ul[0] + 1, 0), D0 - 1);
+    const int a1 = min(max(ul[1], 0), D1 - 1), b1 = min(max(ul[1] + 1, 0), D1 - 1);
+    // _xray3d.py:200-203, in order
+    acc = __fadd_rn(acc, __fmul_rn(y[a0 * D1 + a1], w[0]));
+    acc = __fadd_rn(acc, __fmul_rn(y[b0 * D1 + a1], w[1]));
+    acc = __fadd_rn(acc, __fmul_rn(y[a0 * D1 + b1], w[2]));
+    acc = __fadd_rn(acc, __fmul_rn(y[b0 * D1 + b1], w[3]));
+  }
+  vol[e] = acc;
+}
+
+__global__ __launch_bounds__(256) void gen_fwd_kernel(const float* __restrict__ vol,
+                                                      float* __restrict__ proj,
+                                                      const float* __restrict__ mats, int X,
+                                                      int Y, int Z, int D0, int D1,
+                                                      const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  const long long n = static_cast<long long>(X) * Y * Z;
+  const long long e = blockIdx.x * 256LL + threadIdx.x;
+  if (e >= n) return;
+  const int v = blockIdx.y;
+  const float x = vol[e];
+  if (x == 0.0f) return;
+  const int k = static_cast<int>(e % Z), j = static_cast<int>((e / Z) % Y),
+            i = static_cast<int>(e / (static_cast<long long>(Y) * Z));
+  int ul[2];
+  float w[4];
+  gen_weights(mats + 8 * v, i, j, k, D0, D1, ul, w);
+  float* y = proj + static_cast<size_t>(v) * D0 * D1;
+  if (w[0] != 0.0f) unsafeAtomicAdd(y + ul[0] * D1 + ul[1], __fmul_rn(w[0], x));
+  if (w[1] != 0.0f) unsafeAtomicAdd(y + (ul[0] + 1) * D1 + ul[1], __fmul_rn(w[1], x));
+  if (w[2] != 0.0f) unsafeAtomicAdd(y + ul[0] * D1 + ul[1] + 1, __fmul_rn(w[2], x));
+  if (w[3] != 0.0f) unsafeAtomicAdd(y + (ul[0] + 1) * D1 + ul[1] + 1, __fmul_rn(w[3], x));
+}
+
+__global__ __launch_bounds__(256) void zero_kernel(float* __restrict__ y, long long n,
+                                                   const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += 256LL * gridDim.x) y[e] = 0.0f;
+}
+
+// Reference weight arrays of one view: ul (2, X, Y, Z) int32 and
+// w (4, X, Y, Z) float32 (_xray3d.py:206-266), computed by the same device
+// functions that the projection kernels use.
+__global__ __launch_bounds__(256) void weights_kernel(Plan p, int view, int32_t* __restrict__ ul,
+                                                      float* __restrict__ w) {
+  const long long n = static_cast<long long>(p.X) * p.Y * p.Z;
+  const long long e = blockIdx.x * 256LL + threadIdx.x;
+  if (e >= n) return;
+  const int idx[3] = {static_cast<int>(e / (static_cast<long long>(p.Y) * p.Z)),
+                      static_cast<int>((e / p.Z) % p.Y), static_cast<int>(e % p.Z)};
+  int u[2];
+  float ww[4];
+  if (p.mode == MODE_SLICE) {
+    const float4 g = p.geom[view];
+    const Tap tp = slice_tap(g.x, g.y, g.z, idx[p.pax], idx[p.qax], p.U);
+    const int col = idx[p.sax] + p.delta;
+    const int Dd = p.dax == 0 ? p.D0 : p.D1;
+    const bool cok = col >= 0 && col < Dd;
+    u[p.dax] = col;
+    u[1 - p.dax] = tp.ul;
+    // taps (ul0, ul1), (ul0 + 1, ul1), (ul0, ul1 + 1), (ul0 + 1, ul1 + 1)
+    ww[0] = cok ? tp.wl : 0.0f;
+    ww[3] = 0.0f;
+    if (p.dax == 1) {
+      ww[1] = cok ? tp.wr : 0.0f;
+      ww[2] = 0.0f;
+    } else {
+      ww[1] = 0.0f;
+      ww[2] = cok ? tp.wr : 0.0f;
+    }
+  } else {
+    gen_weights(p.mats + 8 * view, idx[0], idx[1], idx[2], p.D0, p.D1, u, ww);
+  }
+  ul[e] = u[0];
+  ul[n + e] = u[1];
+  for (int t = 0; t < 4; ++t) w[t * n + e] = ww[t];
+}
+
+// ------------------------------------------------------------- host --
+
+static float mat(const std::vector<float>& m, int v, int r, int c) { return m[8 * v + 4 * r + c]; }
+
+// Slice decomposition test for slice axis sax and detector axis dax, all in
+// the reference's float32 arithmetic.
+static bool try_slice(Plan& p, const std::vector<float>& m, int sax, int dax) {
+  const int e = 1 - dax;
+  const int shape[3] = {p.X, p.Y, p.Z};
+  const int S = shape[sax];
+  int delta = 0;
+  for (int v = 0; v < p.V; ++v) {
+    for (int a = 0; a < 3; ++a)
+      if (a != sax && mat(m, v, dax, a) != 0.0f) return false;
+    if (mat(m, v, e, sax) != 0.0f) return false;
+    // Every view must map slice s to column s + delta with t_d = 1/2,
+    // evaluated as the reference does: ((0 + m x) + 0) + t up to exact zeros.
+    const float ms = mat(m, v, dax, sax), td = mat(m, v, dax, 3);
+    if (!(ms != 0.0f)) return false;
+    for (int s = 0; s < S; ++s) {
+      const float x = static_cast<float>(s) + 0.5f;
+      const float prod = ms * x;
+      const float P = prod + td;
+      const float left = P - 0.25f;
+      const float ce = std::ceil(left);
+      const float tt = std::fmin(ce - left, 0.5f);
+      if (tt != 0.5f) return false;
+      const int d = static_cast<int>(std::floor(left)) - s;
+      if (v == 0 && s == 0)
+        delta = d;
+      else if (d != delta)
+        return false;
+    }
+  }
+  int pax = -1, qax = -1;
+  for (int a = 0; a < 3; ++a)
+    if (a != sax) (pax < 0 ? pax : qax) = a;
+  float fmin = INFINITY;
+  for (int v = 0; v < p.V; ++v) {
+    const float a = mat(m, v, e, pax), b = mat(m, v, e, qax), t = mat(m, v, e, 3);
+    if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(t)) return false;
+    fmin = std::fmin(fmin, std::fmax(std::fabs(a), std::fabs(b)));
+  }
+  if (!(fmin > 0.0f)) return false;
+  const double need = std::ceil((FB + 1) / static_cast<double>(fmin)) + 6.0;
+  if (need > 64.0) return false;
+  p.cpr = need <= 32.0 ? 32 : 64;
+  p.mode = MODE_SLICE;
+  p.sax = sax;
+  p.dax = dax;
+  p.delta = delta;
+  p.pax = pax;
+  p.qax = qax;
+  p.P = shape[pax];
+  p.Q = shape[qax];
+  p.S = S;
+  p.SQ = (S + 3) / 4;
+  p.U = e == 0 ? p.D0 : p.D1;
+  p.vol_direct = sax == 2 && S % 4 == 0;
+  p.proj_direct = dax == 1 && delta == 0 && p.D1 == S && S % 4 == 0;
+  return true;
+}
+
+void destroy(Plan& p) {
+  if (p.mats) (void)hipFree(p.mats);
+  if (p.geom) (void)hipFree(p.geom);
+  if (p.volc) (void)hipFree(p.volc);
+  if (p.proja) (void)hipFree(p.proja);
+  p.mats = nullptr;
+  p.geom = nullptr;
+  p.volc = p.proja = nullptr;
+}
+
+static int dev_alloc(void** ptr, size_t bytes) {
+  if (hipMalloc(ptr, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(SCXR_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  return SCXR_OK;
+}
+
+int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D1,
+           int allow_slice) {
+  p.X = X;
+  p.Y = Y;
+  p.Z = Z;
+  p.V = V;
+  p.D0 = D0;
+  p.D1 = D1;
+  std::vector<float> m(mats, mats + 8 * static_cast<size_t>(V));
+  for (float f : m)
+    SCXR_REQUIRE(std::isfinite(f), SCXR_ERR_GEOM, "non-finite projection matrix entry");
+  if (allow_slice) {
+    for (int dax = 1; dax >= 0 && p.mode != MODE_SLICE; --dax)
+      for (int sax = 0; sax < 3 && p.mode != MODE_SLICE; ++sax) try_slice(p, m, sax, dax);
+  }
+  int rc = dev_alloc(reinterpret_cast<void**>(&p.mats), m.size() * sizeof(float));
+  if (rc) return rc;
+  SCXR_HIP(hipMemcpy(p.mats, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+  if (p.mode == MODE_SLICE) {
+    const int e = 1 - p.dax;
+    std::vector<float4> g(V);
+    for (int v = 0; v < V; ++v)
+      g[v] = make_float4(mat(m, v, e, p.pax), mat(m, v, e, p.qax), mat(m, v, e, 3), 0.0f);
+    rc = dev_alloc(reinterpret_cast<void**>(&p.geom), V * sizeof(float4));
+    if (rc) return rc;
+    SCXR_HIP(hipMemcpy(p.geom, g.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    const size_t s4 = 4 * static_cast<size_t>(p.SQ);
+    if (!p.vol_direct) {
+      rc = dev_alloc(reinterpret_cast<void**>(&p.volc),
+                     static_cast<size_t>(p.P) * p.Q * s4 * sizeof(float));
+      if (rc) return rc;
+    }
+    if (!p.proj_direct) {
+      rc = dev_alloc(reinterpret_cast<void**>(&p.proja),
+                     static_cast<size_t>(V) * p.U * s4 * sizeof(float));
+      if (rc) return rc;
+    }
+  }
+  return SCXR_OK;
+}
+
+static int transpose(const float* in, float* out, int B, int Rout, int Cout, int Rin, int Cin,
+                     int roff, int coff, long long ib, long long ir, long long ob, long long oc,
+                     const int32_t* pred, hipStream_t s) {
+  dim3 grid(ceil_div(Cout, 32), ceil_div(Rout, 32), B);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, in, out, Rout, Cout, Rin, Cin, roff,
+                     coff, ib, ir, ob, oc, pred);
+  SCXR_LAUNCHED("xr3d transpose");
+  return SCXR_OK;
+}
+
+static int shift_rows(const float* in, float* out, long long rows, int Cout, int Cin, int off,
+                      long long is, long long os, const int32_t* pred, hipStream_t s) {
+  const long long n = rows * Cout;
+  const unsigned blocks = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 65536));
+  hipLaunchKernelGGL(shift_rows_kernel, dim3(blocks), dim3(256), 0, s, in, out, rows, Cout, Cin,
+                     off, is, os, pred);
+  SCXR_LAUNCHED("xr3d shift_rows");
+  return SCXR_OK;
+}
+
+// vol (X, Y, Z) -> vol_c[p][q][s] (slices padded to 4 SQ with zeros)
+static int vol_to_canon(const Plan& p, const float* vol, float* vc, const int32_t* pred,
+                        hipStream_t s) {
+  const int S4 = 4 * p.SQ;
+  switch (p.sax) {
+    case 2:
+      return shift_rows(vol, vc, static_cast<long long>(p.X) * p.Y, S4, p.S, 0, p.S, S4, pred,
+                        s);
+    case 1:  // vol[p][s][q] -> vc[p][q][s]
+      return transpose(vol, vc, p.X, S4, p.Z, p.S, p.Z, 0, 0,
+                       static_cast<long long>(p.Y) * p.Z, p.Z,
+                       static_cast<long long>(p.Z) * S4, S4, pred, s);
+    default: {  // vol[s][pq] -> vc[pq][s]
+      const int PQ = p.Y * p.Z;
+      return transpose(vol, vc, 1, S4, PQ, p.S, PQ, 0, 0, 0, PQ, 0, S4, pred, s);
+    }
+  }
+}
+
+static int canon_to_vol(const Plan& p, const float* vc, float* vol, const int32_t* pred,
+                        hipStream_t s) {
+  const int S4 = 4 * p.SQ;
+  switch (p.sax) {
+    case 2:
+      return shift_rows(vc, vol, static_cast<long long>(p.X) * p.Y, p.S, S4, 0, S4, p.S, pred,
+                        s);
+    case 1:  // vol[p][s][q] = vc[p][q][s]
+      return transpose(vc, vol, p.X, p.Z, p.S, p.Z, S4, 0, 0,
+                       static_cast<long long>(p.Z) * S4, S4,
+                       static_cast<long long>(p.Y) * p.Z, p.Z, pred, s);
+    default: {  // vol[s][pq] = vc[pq][s]
+      const int PQ = p.Y * p.Z;
+      return transpose(vc, vol, 1, PQ, p.S, PQ, S4, 0, 0, 0, S4, 0, PQ, pred, s);
+    }
+  }
+}
+
+// proj (V, D0, D1) -> proj_c[v][u][s]
+static int proj_to_canon(const Plan& p, const float* y, float* yc, const int32_t* pred,
+                         hipStream_t s) {
+  const int S4 = 4 * p.SQ;
+  if (p.dax == 1)  // proj[v][u][s + delta]
+    return shift_rows(y, yc, static_cast<long long>(p.V) * p.D0, S4, p.D1, p.delta, p.D1, S4,
+                      pred, s);
+  // proj[v][s + delta][u] -> yc[v][u][s]
+  return transpose(y, yc, p.V, S4, p.U, p.D0, p.D1, p.delta, 0,
+                   static_cast<long long>(p.D0) * p.D1, p.D1,
+                   static_cast<long long>(p.U) * S4, S4, pred, s);
+}
+
+static int canon_to_proj(const Plan& p, const float* yc, float* y, const int32_t* pred,
+                         hipStream_t s) {
+  const int S4 = 4 * p.SQ;
+  if (p.dax == 1)  // proj[v][u][c] = yc[v][u][c - delta] for c - delta in [0, S)
+    return shift_rows(yc, y, static_cast<long long>(p.V) * p.D0, p.D1, p.S, -p.delta, S4, p.D1,
+                      pred, s);
+  // proj[v][c][u] = yc[v][u][c - delta]
+  return transpose(yc, y, p.V, p.U, p.D0, p.U, p.S, 0, -p.delta,
+                   static_cast<long long>(p.U) * S4, S4,
+                   static_cast<long long>(p.D0) * p.D1, p.D1, pred, s);
+}
+
+int forward(const Plan& p, const float* vol, float* proj, const int32_t* pred, hipStream_t s) {
+  if (p.mode == MODE_GENERAL) {
+    const long long n = static_cast<long long>(p.X) * p.Y * p.Z;
+    const long long ny = static_cast<long long>(p.V) * p.D0 * p.D1;
+    hipLaunchKernelGGL(zero_kernel, dim3(std::min<long long>((ny + 255) / 256, 65536)), dim3(256),
+                       0, s, proj, ny, pred);
+    SCXR_LAUNCHED("xr3d zero");
+    hipLaunchKernelGGL(gen_fwd_kernel, dim3(ceil_div(n, 256), p.V), dim3(256), 0, s, vol, proj,
+                       p.mats, p.X, p.Y, p.Z, p.D0, p.D1, pred);
+    SCXR_LAUNCHED("xr3d general forward");
+    return SCXR_OK;
+  }
+  const float* vc = vol;
+  if (!p.vol_direct) {
+    int rc = vol_to_canon(p, vol, p.volc, pred, s);
+    if (rc) return rc;
+    vc = p.volc;
+  }
+  float* yc = p.proj_direct ? proj : p.proja;
+  dim3 grid(ceil_div(p.U, FB), ceil_div(p.V, 4), ceil_div(p.SQ, 64));
+  if (p.cpr == 32)
+    hipLaunchKernelGGL(fwd_slice_kernel<32>, grid, dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(vc), reinterpret_cast<float4*>(yc), p.geom,
+                       p.P, p.Q, p.SQ, p.U, p.V, pred);
+  else
+    hipLaunchKernelGGL(fwd_slice_kernel<64>, grid, dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(vc), reinterpret_cast<float4*>(yc), p.geom,
+                       p.P, p.Q, p.SQ, p.U, p.V, pred);
+  SCXR_LAUNCHED("xr3d slice forward");
+  if (!p.proj_direct) return canon_to_proj(p, yc, proj, pred, s);
+  return SCXR_OK;
+}
+
+int adjoint(const Plan& p, const float* proj, float* vol, const int32_t* pred, hipStream_t s) {
+  if (p.mode == MODE_GENERAL) {
+    const long long n = static_cast<long long>(p.X) * p.Y * p.Z;
+    hipLaunchKernelGGL(gen_adj_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, proj, vol, p.mats,
+                       p.X, p.Y, p.Z, p.V, p.D0, p.D1, pred);
+    SCXR_LAUNCHED("xr3d general adjoint");
+    return SCXR_OK;
+  }
+  const float* yc = proj;
+  if (!p.proj_direct) {
+    int rc = proj_to_canon(p, proj, p.proja, pred, s);
+    if (rc) return rc;
+    yc = p.proja;
+  }
+  float* vc = p.vol_direct ? vol : p.volc;
+  dim3 grid(ceil_div(p.Q, 8), ceil_div(p.P, 4), ceil_div(p.SQ, 64));
+  hipLaunchKernelGGL(adj_slice_kernel, grid, dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(yc), reinterpret_cast<float4*>(vc), p.geom,
+                     p.P, p.Q, p.SQ, p.U, p.V, pred);
+  SCXR_LAUNCHED("xr3d slice adjoint");
+  if (!p.vol_direct) return canon_to_vol(p, vc, vol, pred, s);
+  return SCXR_OK;
+}
+
+}  // namespace xr3d
+}  // namespace scxr
+
+struct scxr_xr3d_plan {
+  scxr::xr3d::Plan p;
+};
+
+using namespace scxr;
+
+extern "C" {
+
+int scxr_xr3d_plan_create(int32_t X, int32_t Y, int32_t Z, int32_t V, const float* matrices,
+                          int32_t D0, int32_t D1, int32_t allow_slice, scxr_xr3d_plan** out) {
+  SCXR_REQUIRE(out && matrices, SCXR_ERR_ARG, "null argument to xr3d_plan_create");
+  SCXR_REQUIRE(X > 0 && Y > 0 && Z > 0 && V > 0 && D0 > 0 && D1 > 0, SCXR_ERR_SHAPE,
+               "non-positive size (X=%d Y=%d Z=%d V=%d D0=%d D1=%d)", X, Y, Z, V, D0, D1);
+  SCXR_REQUIRE(static_cast<long long>(X) * Y * Z < (1LL << 31) &&
+                   static_cast<long long>(D0) * D1 < (1LL << 31),
+               SCXR_ERR_SHAPE, "volume or detector too large");
+  auto* plan = new scxr_xr3d_plan();
+  int rc = xr3d::create(plan->p, X, Y, Z, V, matrices, D0, D1, allow_slice);
+  if (rc != SCXR_OK) {
+    xr3d::destroy(plan->p);
+    delete plan;
+    return rc;
+  }
+  *out = plan;
+  return SCXR_OK;
+}
+
+int scxr_xr3d_plan_destroy(scxr_xr3d_plan* plan) {
+  if (!plan) return SCXR_OK;
+  xr3d::destroy(plan->p);
+  delete plan;
+  return SCXR_OK;
+}
+
+int scxr_xr3d_info(const scxr_xr3d_plan* plan, int32_t* out) {
+  SCXR_REQUIRE(plan && out, SCXR_ERR_ARG, "null argument to xr3d_info");
+  const auto& p = plan->p;
+  out[0] = p.mode;
+  out[1] = p.sax;
+  out[2] = p.dax;
+  out[3] = p.delta;
+  out[4] = p.cpr;
+  out[5] = p.vol_direct;
+  out[6] = p.proj_direct;
+  out[7] = p.U;
+  return SCXR_OK;
+}
+
+int scxr_xr3d_forward(const scxr_xr3d_plan* plan, const float* vol, float* proj,
+                      const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(plan && vol && proj, SCXR_ERR_ARG, "null argument to xr3d_forward");
+  return xr3d::forward(plan->p, vol, proj, pred, as_stream(stream));
+}
+
+int scxr_xr3d_adjoint(const scxr_xr3d_plan* plan, const float* proj, float* vol,
+                      const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(plan && vol && proj, SCXR_ERR_ARG, "null argument to xr3d_adjoint");
+  return xr3d::adjoint(plan->p, proj, vol, pred, as_stream(stream));
+}
+
+int scxr_xr3d_weights(const scxr_xr3d_plan* plan, int32_t view, int32_t* ul, float* w,
+                      void* stream) {
+  SCXR_REQUIRE(plan && ul && w, SCXR_ERR_ARG, "null argument to xr3d_weights");
+  const auto& p = plan->p;
+  SCXR_REQUIRE(view >= 0 && view < p.V, SCXR_ERR_ARG, "view %d out of range", view);
+  const long long n = static_cast<long long>(p.X) * p.Y * p.Z;
+  hipLaunchKernelGGL(xr3d::weights_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                     as_stream(stream), p, view, ul, w);
+  SCXR_LAUNCHED("xr3d weights");
+  return SCXR_OK;
+}
+
+}  // extern "C"
