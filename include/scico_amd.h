@@ -215,6 +215,22 @@ int scxr_admm_zu2d(int32_t M, int32_t N, const float* x, float* z, float* u, flo
 /* Partials of ||rho * C^T (z - zold)||^2 (dual residual, _admm.py:279-295). */
 int scxr_admm_dual2d(int32_t M, int32_t N, const float* z, const float* zold, float rho,
                      int32_t circular, double* ws, void* stream);
+/* nd variants (ndim <= 3, FiniteDifference over all axes) for operators
+ * without projector epilogues (XRayTransform3D, ...):
+ *   out = base + rho C^T (z - u)              (_admmaux.py:245-256)
+ *   out = rho C^T C x + ata, partials of x.out (_admmaux.py:258-268)
+ * out may alias base / ata. */
+int scxr_admm_rhs_nd(int32_t ndim, const int32_t* shape /* host */, const float* base,
+                     const float* z, const float* u, float rho, int32_t circular, float* out,
+                     void* stream);
+int scxr_admm_lhs_nd(int32_t ndim, const int32_t* shape /* host */, const float* x,
+                     const float* ata, float rho, int32_t circular, float* out, double* dot_ws,
+                     const int32_t* pred, void* stream);
+int scxr_admm_zu_nd(int32_t ndim, const int32_t* shape /* host */, const float* x, float* z,
+                    float* u, float* zold, float thr, int32_t circular, int32_t isotropic,
+                    double* ws_prml, double* ws_g, void* stream);
+int scxr_admm_dual_nd(int32_t ndim, const int32_t* shape /* host */, const float* z,
+                      const float* zold, float rho, int32_t circular, double* ws, void* stream);
 /* Appends (Objective, Prml Rsdl, Dual Rsdl, CG It, CG Res) to hist[5 * *counter]
  * (scico/optimize/_admm.py:171-198, 241-295) and increments *counter. */
 int scxr_admm_itstat(const double* res, int32_t nres, const double* g, int32_t ng,

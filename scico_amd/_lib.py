@@ -94,6 +94,16 @@ SIGNATURES = [
       c_void_p, c_void_p, c_void_p]),
     ("scxr_admm_dual2d", c_int32,
      [c_int32, c_int32, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
+    ("scxr_admm_rhs_nd", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
+    ("scxr_admm_lhs_nd", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p, c_void_p,
+      c_void_p]),
+    ("scxr_admm_zu_nd", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32,
+      c_void_p, c_void_p, c_void_p]),
+    ("scxr_admm_dual_nd", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
     ("scxr_admm_itstat", c_int32,
      [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
       c_double, c_double, c_void_p, c_void_p, c_int32, c_void_p]),

@@ -320,6 +320,178 @@ __global__ __launch_bounds__(256) void admm_dual2d_kernel(int M, int N,
   if (threadIdx.x == 0) ws[blockIdx.x] = s;
 }
 
+// ------------------------------------------------- nd ADMM-TV (any A) --
+// Fused passes of the ADMM-TV iteration for an operator A without
+// projector epilogues (3D transform, ...), over all axes of an
+// ndim-D array (ndim <= 3, extents padded to 3 with leading 1s).
+
+struct NdIdx {
+  int dims[3];
+  int64_t st[3];
+  int64_t n;
+  __device__ NdIdx(int d0, int d1, int d2) {
+    dims[0] = d0;
+    dims[1] = d1;
+    dims[2] = d2;
+    st[0] = static_cast<int64_t>(d1) * d2;
+    st[1] = d2;
+    st[2] = 1;
+    n = static_cast<int64_t>(d0) * d1 * d2;
+  }
+  __device__ void split(int64_t e, int idx[3]) const {
+    idx[0] = static_cast<int>(e / st[0]);
+    idx[1] = static_cast<int>((e / dims[2]) % dims[1]);
+    idx[2] = static_cast<int>(e % dims[2]);
+  }
+};
+
+// (C^T w)[e] for w = a - b stacked (ndim, n) (b may be null), axis order sum.
+__device__ __forceinline__ float adj_nd(const float* __restrict__ a, const float* __restrict__ b,
+                                        const NdIdx& t, int ndim, int64_t e, const int idx[3],
+                                        bool cc) {
+  float acc = 0.0f;
+  for (int c = 0; c < ndim; ++c) {
+    const int ax = 3 - ndim + c;
+    const int k = idx[ax], len = t.dims[ax];
+    const int km = (k >= 1) ? k - 1 : len - 1;
+    const int64_t em = e + static_cast<int64_t>(km - k) * t.st[ax];
+    const int64_t o = c * t.n;
+    const float wm = b ? __fsub_rn(a[o + em], b[o + em]) : a[o + em];
+    const float w0 = b ? __fsub_rn(a[o + e], b[o + e]) : a[o + e];
+    const float v = fd::diff_adj(wm, w0, k, len, cc);
+    acc = (c == 0) ? v : __fadd_rn(acc, v);
+  }
+  return acc;
+}
+
+// out = base + rho C^T (z - u)   (_admmaux.py:245-256); out may alias base.
+__global__ __launch_bounds__(256) void admm_rhs_nd_kernel(int ndim, int d0, int d1, int d2,
+                                                          const float* base,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ u, float rho,
+                                                          int circ, float* out) {
+  const NdIdx t(d0, d1, d2);
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
+    int idx[3];
+    t.split(e, idx);
+    const float c = __fmul_rn(rho, adj_nd(z, u, t, ndim, e, idx, circ != 0));
+    out[e] = __fadd_rn(base[e], c);
+  }
+}
+
+// out = rho C^T C x + ata (out may alias ata), with per-block partials of
+// sum x * out when dot_ws is given (_admmaux.py:258-268, solver.py:392).
+__global__ __launch_bounds__(256) void admm_lhs_nd_kernel(int ndim, int d0, int d1, int d2,
+                                                          const float* __restrict__ x,
+                                                          const float* ata, float rho, int circ,
+                                                          float* out, double* __restrict__ dot_ws,
+                                                          const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ double sh[4];
+  const NdIdx t(d0, d1, d2);
+  const bool cc = circ != 0;
+  double dot = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
+    int idx[3];
+    t.split(e, idx);
+    const float xc = x[e];
+    float acc = 0.0f;
+    for (int c = 0; c < ndim; ++c) {
+      const int ax = 3 - ndim + c;
+      const int k = idx[ax], len = t.dims[ax];
+      const int kp = (k + 1 < len) ? k + 1 : 0;
+      const int km = (k >= 1) ? k - 1 : len - 1;
+      const int64_t st = t.st[ax];
+      const float g0 = fd::diff_fwd(xc, x[e + (kp - k) * st], k, len, cc);
+      const float gm = fd::diff_fwd(x[e + (km - k) * st], xc, km, len, cc);
+      const float v = fd::diff_adj(gm, g0, k, len, cc);
+      acc = (c == 0) ? v : __fadd_rn(acc, v);
+    }
+    const float o = __fadd_rn(__fmul_rn(rho, acc), ata[e]);
+    out[e] = o;
+    dot += static_cast<double>(xc) * o;
+  }
+  if (dot_ws) {
+    dot = block_sum_f64(dot, sh);
+    if (threadIdx.x == 0) dot_ws[blockIdx.x] = dot;
+  }
+}
+
+// z/u update of _admm.py:364-378 with C = FiniteDifference over all axes:
+//   Cx = C x, v = Cx + u, z = prox(v, thr), u = u + Cx - z, zold = z_prev,
+// plus per-block partials of ||Cx - z||^2 and of g(z) (L21 over the
+// stack axis, or L1).
+__global__ __launch_bounds__(256) void admm_zu_nd_kernel(int ndim, int d0, int d1, int d2,
+                                                         const float* __restrict__ x,
+                                                         float* __restrict__ z,
+                                                         float* __restrict__ u,
+                                                         float* __restrict__ zold, float thr,
+                                                         int circ, int iso,
+                                                         double* __restrict__ ws_prml,
+                                                         double* __restrict__ ws_g) {
+  __shared__ double sh[4];
+  const NdIdx t(d0, d1, d2);
+  const bool cc = circ != 0;
+  double prml = 0.0, gsum = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
+    int idx[3];
+    t.split(e, idx);
+    const float xc = x[e];
+    float cx[3], v[3], uo[3];
+    for (int c = 0; c < ndim; ++c) {
+      const int ax = 3 - ndim + c;
+      const int k = idx[ax], len = t.dims[ax];
+      const int kn = (k + 1 < len) ? k + 1 : 0;
+      cx[c] = fd::diff_fwd(xc, x[e + static_cast<int64_t>(kn - k) * t.st[ax]], k, len, cc);
+      const int64_t o = c * t.n + e;
+      zold[o] = z[o];
+      uo[c] = u[o];
+      v[c] = __fadd_rn(cx[c], uo[c]);
+    }
+    if (iso) {
+      l21_shrink(v, ndim, thr);
+      double ss = 0.0;
+      for (int c = 0; c < ndim; ++c) ss += static_cast<double>(v[c]) * v[c];
+      gsum += sqrt(ss);
+    } else {
+      for (int c = 0; c < ndim; ++c) {
+        v[c] = l1_shrink(v[c], thr);
+        gsum += fabs(static_cast<double>(v[c]));
+      }
+    }
+    for (int c = 0; c < ndim; ++c) {
+      const int64_t o = c * t.n + e;
+      z[o] = v[c];
+      u[o] = __fsub_rn(__fadd_rn(uo[c], cx[c]), v[c]);
+      const double d = static_cast<double>(__fsub_rn(cx[c], v[c]));
+      prml += d * d;
+    }
+  }
+  prml = block_sum_f64(prml, sh);
+  if (threadIdx.x == 0) ws_prml[blockIdx.x] = prml;
+  gsum = block_sum_f64(gsum, sh);
+  if (threadIdx.x == 0) ws_g[blockIdx.x] = gsum;
+}
+
+// Partials of ||rho C^T (z - zold)||^2 (dual residual, _admm.py:279-295).
+__global__ __launch_bounds__(256) void admm_dual_nd_kernel(int ndim, int d0, int d1, int d2,
+                                                           const float* __restrict__ z,
+                                                           const float* __restrict__ zold,
+                                                           float rho, int circ,
+                                                           double* __restrict__ ws) {
+  __shared__ double sh[4];
+  const NdIdx t(d0, d1, d2);
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
+    int idx[3];
+    t.split(e, idx);
+    const float v = __fmul_rn(rho, adj_nd(z, zold, t, ndim, e, idx, circ != 0));
+    s += static_cast<double>(v) * v;
+  }
+  s = block_sum_f64(s, sh);
+  if (threadIdx.x == 0) ws[blockIdx.x] = s;
+}
+
 // Iteration statistics row (Objective, Prml Rsdl, Dual Rsdl, CG It, CG Res)
 // appended at hist[5 * counter]; counter is a device int.
 __global__ __launch_bounds__(256) void admm_itstat_kernel(
@@ -579,6 +751,73 @@ int scxr_admm_dual2d(int32_t M, int32_t N, const float* z, const float* zold, fl
   hipLaunchKernelGGL(solver::admm_dual2d_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                      as_stream(stream), M, N, z, zold, rho, circular, ws);
   SCXR_LAUNCHED("admm dual2d");
+  return SCXR_OK;
+}
+
+static int nd_dims(int32_t ndim, const int32_t* shape, int d[3]) {
+  SCXR_REQUIRE(ndim >= 1 && ndim <= 3 && shape, SCXR_ERR_ARG, "bad nd shape (ndim in 1..3)");
+  d[0] = d[1] = d[2] = 1;
+  for (int k = 0; k < ndim; ++k) {
+    SCXR_REQUIRE(shape[k] > 0, SCXR_ERR_SHAPE, "empty array");
+    d[3 - ndim + k] = shape[k];
+  }
+  return SCXR_OK;
+}
+
+int scxr_admm_rhs_nd(int32_t ndim, const int32_t* shape, const float* base, const float* z,
+                     const float* u, float rho, int32_t circular, float* out, void* stream) {
+  int d[3];
+  int rc = nd_dims(ndim, shape, d);
+  if (rc) return rc;
+  SCXR_REQUIRE(base && z && u && out, SCXR_ERR_ARG, "bad admm_rhs_nd arguments");
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  hipLaunchKernelGGL(solver::admm_rhs_nd_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], base, z, u, rho, circular, out);
+  SCXR_LAUNCHED("admm rhs nd");
+  return SCXR_OK;
+}
+
+int scxr_admm_lhs_nd(int32_t ndim, const int32_t* shape, const float* x, const float* ata,
+                     float rho, int32_t circular, float* out, double* dot_ws,
+                     const int32_t* pred, void* stream) {
+  int d[3];
+  int rc = nd_dims(ndim, shape, d);
+  if (rc) return rc;
+  SCXR_REQUIRE(x && ata && out, SCXR_ERR_ARG, "bad admm_lhs_nd arguments");
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  hipLaunchKernelGGL(solver::admm_lhs_nd_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], x, ata, rho, circular, out,
+                     dot_ws, pred);
+  SCXR_LAUNCHED("admm lhs nd");
+  return SCXR_OK;
+}
+
+int scxr_admm_zu_nd(int32_t ndim, const int32_t* shape, const float* x, float* z, float* u,
+                    float* zold, float thr, int32_t circular, int32_t isotropic,
+                    double* ws_prml, double* ws_g, void* stream) {
+  int d[3];
+  int rc = nd_dims(ndim, shape, d);
+  if (rc) return rc;
+  SCXR_REQUIRE(x && z && u && zold && ws_prml && ws_g, SCXR_ERR_ARG,
+               "bad admm_zu_nd arguments");
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  hipLaunchKernelGGL(solver::admm_zu_nd_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], x, z, u, zold, thr, circular,
+                     isotropic, ws_prml, ws_g);
+  SCXR_LAUNCHED("admm zu nd");
+  return SCXR_OK;
+}
+
+int scxr_admm_dual_nd(int32_t ndim, const int32_t* shape, const float* z, const float* zold,
+                      float rho, int32_t circular, double* ws, void* stream) {
+  int d[3];
+  int rc = nd_dims(ndim, shape, d);
+  if (rc) return rc;
+  SCXR_REQUIRE(z && zold && ws, SCXR_ERR_ARG, "bad admm_dual_nd arguments");
+  const int64_t n = static_cast<int64_t>(d[0]) * d[1] * d[2];
+  hipLaunchKernelGGL(solver::admm_dual_nd_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), ndim, d[0], d[1], d[2], z, zold, rho, circular, ws);
+  SCXR_LAUNCHED("admm dual nd");
   return SCXR_OK;
 }
 

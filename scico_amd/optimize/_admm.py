@@ -8,10 +8,12 @@ tomographic ADMM-TV structure
     g = lam * L21Norm() or lam * L1Norm(),   C = FiniteDifference (2D),
     LinearSubproblemSolver (scico CG),       alpha = 1,
 
-the same iteration runs through fused kernels (:class:`_FusedTV2D`): the
+the same iteration runs through fused kernels (:class:`_FusedTV`): the
 x-step operator ``rho C^T C + A^T A`` and the right-hand side are fused
 into the back projection, the CG scalars and stopping test stay on the
 device, and the z/u update and iteration statistics are single passes.
+With ``XRayTransform3D`` (3D, FiniteDifference over all axes) the same
+structure runs with the FD terms in separate nd passes.
 """
 
 from __future__ import annotations
@@ -27,7 +29,7 @@ from .. import _vec as vec
 from .._lib import call, load
 from ..functional import L1Norm, L21Norm, ScaledFunctional
 from ..linop import FiniteDifference
-from ..linop.xray import XRayTransform2D
+from ..linop.xray import XRayTransform2D, XRayTransform3D
 from ..loss import SquaredL2Loss
 from ..solver import CGWork
 from ._admmaux import GenericSubproblemSolver, LinearSubproblemSolver
@@ -57,7 +59,7 @@ class ADMM(Optimizer):
         self.z_list, self.z_list_old = self.z_init(self.x)
         self.u_list = self.u_init(self.x)
         super().__init__(**kwargs)
-        self._fused = _FusedTV2D.build(self)
+        self._fused = _FusedTV.build(self)
 
     # --- reference API
     def _working_vars_finite(self) -> bool:
@@ -162,57 +164,64 @@ def _tv_functional(g):
     return None
 
 
-class _FusedTV2D:
-    """Fused device implementation of ADMM-TV with the 2D projector."""
+class _FusedTV:
+    """Fused device implementation of ADMM-TV with an X-ray transform."""
 
     enabled = True  # tests switch this off to run the generic path
 
     @staticmethod
     def build(admm: ADMM):
-        if not _FusedTV2D.enabled:
+        if not _FusedTV.enabled:
             return None
         ss = admm.subproblem_solver
         if type(ss) is not LinearSubproblemSolver or ss.cg_function != "scico":
             return None
         f = admm.f
-        if not (isinstance(f, SquaredL2Loss) and isinstance(f.A, XRayTransform2D)
+        if not (isinstance(f, SquaredL2Loss) and isinstance(f.A, (XRayTransform2D, XRayTransform3D))
                 and f.scale == 0.5):
             return None
         if len(admm.C_list) != 1 or admm.alpha != 1.0:
             return None
         C = admm.C_list[0]
-        if not (type(C) is FiniteDifference and len(C.input_shape) == 2
-                and C.input_shape == f.A.input_shape):
+        if not (type(C) is FiniteDifference and C.input_shape == f.A.input_shape):
             return None
         tv = _tv_functional(admm.g_list[0])
         if tv is None:
             return None
         if float(ss.cg_kwargs.get("atol", 0.0)) < 0 or set(ss.cg_kwargs) - {"tol", "atol", "maxiter"}:
             return None
-        return _FusedTV2D(admm, tv[0], tv[1])
+        return _FusedTV(admm, tv[0], tv[1])
 
     def __init__(self, admm: ADMM, lam: float, isotropic: bool):
         self.admm = admm
-        self.A: XRayTransform2D = admm.f.A
+        self.A = admm.f.A
         self.C: FiniteDifference = admm.C_list[0]
         self.lam = float(lam)
         self.iso = bool(isotropic)
         self.rho = float(admm.rho_list[0])
         self.circ = int(self.C.circular)
+        # 2D projector: FD terms fused into the projector epilogues
+        self.epi = isinstance(self.A, XRayTransform2D)
         cgk = admm.subproblem_solver.cg_kwargs
         self.tol = float(cgk.get("tol", 1e-4))
         self.atol = float(cgk.get("atol", 0.0))
         self.cg_maxiter = int(cgk.get("maxiter", 100))
-        self.M, self.N = self.A.input_shape
-        n = self.M * self.N
+        shape = self.A.input_shape
+        self.ndim = len(shape)
+        self.shape_c = (ctypes.c_int32 * self.ndim)(*shape)
+        n = int(np.prod(shape))
         lib = load()
-        info = (ctypes.c_int32 * 4)()
-        call("scxr_xr2d_info", self.A.plan, info)
-        self.n_fwd, self.n_adj = int(info[0]), int(info[1])
         self.G = int(lib.scxr_elem_blocks(n))
+        if self.epi:
+            self.M, self.N = shape
+            info = (ctypes.c_int32 * 4)()
+            call("scxr_xr2d_info", self.A.plan, info)
+            self.n_fwd, self.n_adj = int(info[0]), int(info[1])
+        else:
+            self.n_fwd, self.n_adj = 1, self.G
         f64 = dict(dtype=torch.float64, device=dev.DEVICE)
         self.cg = CGWork(n, self.cg_maxiter)
-        self.r = torch.empty((self.M, self.N), dtype=torch.float32, device=dev.DEVICE)
+        self.r = torch.empty(shape, dtype=torch.float32, device=dev.DEVICE)
         self.p = torch.empty_like(self.r)
         self.Ap = torch.empty_like(self.r)
         self.rhs = torch.empty_like(self.r)
@@ -286,6 +295,9 @@ class _FusedTV2D:
 
     def enqueue(self):
         """Enqueue one full ADMM iteration (no host synchronisation; capturable)."""
+        if not self.epi:
+            self._enqueue_nd()
+            return
         A, plan, s = self.A, self.A.plan, dev.stream()
         P = lambda t: t.data_ptr()
         y = self.admm.f.y
@@ -311,6 +323,40 @@ class _FusedTV2D:
              P(self.ws_dual), s)
         call("scxr_xr2d_forward_res", plan, P(self.x), P(self.sino), P(y), P(self.ws_res), None,
              s)
+        self._itstat()
+
+    def _itstat(self):
+        P = lambda t: t.data_ptr()
         call("scxr_admm_itstat", P(self.ws_res), self.n_fwd, P(self.ws_g), self.G,
-             P(self.ws_prml), self.G, P(self.ws_dual), self.G, P(cg.state), self.lam, self.rho,
-             P(self.hist), P(self.counter), self.hist_cap, s)
+             P(self.ws_prml), self.G, P(self.ws_dual), self.G, P(self.cg.state), self.lam,
+             self.rho, P(self.hist), P(self.counter), self.hist_cap, dev.stream())
+
+    def _enqueue_nd(self):
+        """The same iteration for any projector with ``project`` /
+        ``back_project(out=, pred=)``; FD terms in nd passes."""
+        A, s, nd, sh = self.A, dev.stream(), self.ndim, self.shape_c
+        P = lambda t: t.data_ptr()
+        y = self.admm.f.y
+        A.back_project(y, out=self.rhs)
+        call("scxr_admm_rhs_nd", nd, sh, P(self.rhs), P(self.z), P(self.u), self.rho, self.circ,
+             P(self.rhs), s)
+        A.project(self.x, out=self.sino)
+        A.back_project(self.sino, out=self.Ap)
+        call("scxr_admm_lhs_nd", nd, sh, P(self.x), P(self.Ap), self.rho, self.circ, P(self.Ap),
+             None, None, s)
+        cg = self.cg
+        cg.start(self.rhs, self.Ap, self.r, self.p, self.tol, self.atol)
+        for k in range(self.cg_maxiter):
+            flag = cg.flags[k : k + 1]
+            A.project(self.p, out=self.sino, pred=flag)
+            A.back_project(self.sino, out=self.Ap, pred=flag)
+            call("scxr_admm_lhs_nd", nd, sh, P(self.p), P(self.Ap), self.rho, self.circ,
+                 P(self.Ap), P(self.pap), P(flag), s)
+            cg.iterate(k, self.x, self.r, self.p, self.Ap, self.pap, self.n_adj)
+        call("scxr_admm_zu_nd", nd, sh, P(self.x), P(self.z), P(self.u), P(self.zold), self.thr,
+             self.circ, int(self.iso), P(self.ws_prml), P(self.ws_g), s)
+        call("scxr_admm_dual_nd", nd, sh, P(self.z), P(self.zold), self.rho, self.circ,
+             P(self.ws_dual), s)
+        A.project(self.x, out=self.sino)
+        vec.reduce(vec.RED_SQDIST, self.sino, y, out=self.ws_res[0])
+        self._itstat()

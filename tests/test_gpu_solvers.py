@@ -95,8 +95,8 @@ def _oracle_admm(g, y, lam, rho, iters, cg_maxiter, circular=False, iso=True, x0
 
 def _gpu_solver(ang, y, N, D, lam, rho, iters, cg_maxiter, circular=False, iso=True, fused=True,
                 x0=None):
-    old = admm_mod._FusedTV2D.enabled
-    admm_mod._FusedTV2D.enabled = fused
+    old = admm_mod._FusedTV.enabled
+    admm_mod._FusedTV.enabled = fused
     try:
         A = XRayTransform2D((N, N), ang, det_count=D)
         f = loss.SquaredL2Loss(y=dev(y), A=A)
@@ -109,7 +109,7 @@ def _gpu_solver(ang, y, N, D, lam, rho, iters, cg_maxiter, circular=False, iso=T
         assert (solver._fused is not None) == fused
         return solver
     finally:
-        admm_mod._FusedTV2D.enabled = old
+        admm_mod._FusedTV.enabled = old
 
 
 def _gpu_admm(ang, y, N, D, lam, rho, iters, cg_maxiter, circular=False, iso=True, fused=True):
