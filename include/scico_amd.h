@@ -249,6 +249,19 @@ int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out,
                  int32_t isotropic, int32_t circular, const int32_t* pred, void* stream);
 /* TV value norm(FD(x)) (scico/functional/_tvnorm.py:89-115) into the device
  * double *out; ws: float64[scxr_elem_blocks(n) + 1]. */
+/* tv_prox with a per-pixel weight map: threshold tau * w[p] at coefficient
+ * location p (w nullable = all ones).  Used by the differentiable unrolled
+ * reconstruction; no reference counterpart beyond TVNorm.prox with w = 1. */
+int scxr_tv_prox_w(int32_t ndim, const int32_t* shape /* host */, const float* v,
+                   const float* w, float* out, float tau, int32_t isotropic,
+                   int32_t circular, const int32_t* pred, void* stream);
+/* Float count of the workspace of scxr_tv_prox_vjp (2 ndim prod(padded shape)). */
+int64_t scxr_tv_prox_vjp_ws(int32_t ndim, const int32_t* shape, int32_t circular);
+/* Vector-Jacobian product of scxr_tv_prox_w: given the cotangent gout of
+ * out, gv = (d out / d v)^T gout and gw = (d out / d w)^T gout (gw nullable). */
+int scxr_tv_prox_vjp(int32_t ndim, const int32_t* shape /* host */, const float* v,
+                     const float* w, const float* gout, float tau, int32_t isotropic,
+                     int32_t circular, float* gv, float* gw, float* ws, void* stream);
 int scxr_tv_norm(int32_t ndim, const int32_t* shape, const float* x, int32_t isotropic,
                  int32_t circular, double* out, double* ws, const int32_t* pred, void* stream);
 

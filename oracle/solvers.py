@@ -169,6 +169,81 @@ def tv_prox(v, lam, isotropic=True, circular=False):
     return (f32(1.0 / K) * u).astype(f32)
 
 
+def tv_prox_weighted(v, lam, weight=None, isotropic=True, circular=False, dtype=f32):
+    """:func:`tv_prox` with a per-pixel threshold map ``thr * weight`` on the
+    high-pass coefficients at each unpadded location (``weight=None`` is the
+    reference prox, ``_tvnorm.py:183-236``).  ``dtype=np.float64`` gives the
+    float64 twin used for finite-difference Jacobians.
+    """
+    dt = np.dtype(dtype)
+    v = np.asarray(v, dtype=dt)
+    nd = v.ndim
+    K = 2 * nd
+    s = dt.type(1.0 / np.sqrt(2.0))
+    if dt == np.float32:
+        thr = f32(np.sqrt(2.0)) * f32(K) * f32(lam)
+    else:
+        thr = np.sqrt(2.0) * K * float(lam)
+    tmap = thr if weight is None else (dt.type(thr) * np.asarray(weight, dtype=dt)).astype(dt)
+
+    def W(x):
+        out = []
+        for ax in range(nd):
+            r = np.roll(x, -1, axis=ax)
+            out.append(np.stack([s * (x + r), s * (r - x)]))
+        return np.stack(out).astype(dt)
+
+    def WT(w):
+        total = None
+        for ax in range(nd):
+            lo, hi = w[ax, 0], w[ax, 1]
+            t = s * (lo + np.roll(lo, 1, axis=ax)) + s * (np.roll(hi, 1, axis=ax) - hi)
+            total = t if total is None else total + t
+        return total.astype(dt)
+
+    def shrink(hp):
+        if isotropic:
+            ln = np.sqrt(np.sum(hp * hp, axis=0))
+            nl = dt.type(0.5) * ((ln - tmap) + np.abs(ln - tmap))
+            with np.errstate(invalid="ignore", divide="ignore"):
+                d = np.where(ln != 0, hp / np.where(ln != 0, ln, 1), 0)
+            return (nl * d).astype(dt)
+        t = np.abs(hp) - tmap
+        return (np.sign(hp) * (dt.type(0.5) * (t + np.abs(t)))).astype(dt)
+
+    if circular:
+        w = W(v)
+        w[:, 1] = shrink(w[:, 1])
+        return (dt.type(1.0 / K) * WT(w)).astype(dt)
+    vp = np.pad(v, [(0, 1)] * nd, mode="edge")
+    w = W(vp)
+    inner = (slice(None), 1) + tuple(slice(0, -1) for _ in range(nd))
+    w[inner] = shrink(w[inner])
+    u = WT(w)[tuple(slice(0, -1) for _ in range(nd))]
+    return (dt.type(1.0 / K) * u).astype(dt)
+
+
+def unrolled_apgm_tv(y, A, AT, L, lam, weight, K, x0, dtype=f32):
+    """``K`` AcceleratedPGM steps for ``0.5 ||A x - y||^2 + lam TV_w(x)`` with
+    the weighted Haar TV prox (:func:`tv_prox_weighted`), constant step
+    ``1/L`` (``_pgm.py:221-240``); the momentum sequence is the float32
+    one of ``AcceleratedPGM``.  Returns ``x_K``."""
+    dt = np.dtype(dtype)
+    x = np.asarray(x0, dtype=dt)
+    v = x
+    t = 1.0
+    gam = 1.0 / L
+    for _ in range(K):
+        x_old = x
+        u = (v - dt.type(gam) * AT((A(v) - y).astype(dt))).astype(dt)
+        x = tv_prox_weighted(u, gam * lam, weight, dtype=dt)
+        t_old = t
+        t = float(f32(0.5) * (f32(1) + np.sqrt(f32(1) + f32(4) * f32(t_old) ** 2)))
+        c = float(f32(f32(t_old) - f32(1)) / f32(t))
+        v = (x + dt.type(c) * (x - x_old)).astype(dt)
+    return x
+
+
 def tv_norm(x, isotropic=True, circular=False):
     """``TVNorm.__call__`` = norm(FD(x)) (``_tvnorm.py:89-115``)."""
     g = fd(x, circular=circular)

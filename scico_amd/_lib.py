@@ -110,6 +110,13 @@ SIGNATURES = [
     # TV
     ("scxr_tv_prox", c_int32,
      [c_int32, c_int32_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_void_p, c_void_p]),
+    ("scxr_tv_prox_w", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_void_p,
+      c_void_p]),
+    ("scxr_tv_prox_vjp_ws", c_int64, [c_int32, c_int32_p, c_int32]),
+    ("scxr_tv_prox_vjp", c_int32,
+     [c_int32, c_int32_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_void_p,
+      c_void_p, c_void_p, c_void_p]),
     ("scxr_tv_norm", c_int32,
      [c_int32, c_int32_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     # CircularConvolve / FFT

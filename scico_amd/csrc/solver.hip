@@ -544,10 +544,11 @@ __device__ __forceinline__ float tv_vp(const float* __restrict__ v, const TvDims
 }
 
 // Haar high-pass coefficients of all axes at q (shrunk if inside the
-// region), and the low-pass of axis `la` at q.
+// region, threshold tau * w[q] when a weight map is given), and the
+// low-pass of every axis at q.
 __device__ __forceinline__ void tv_coeffs(const float* __restrict__ v, const TvDims& t,
-                                          const int* q, float tau, int iso, float* H,
-                                          float* Lo) {
+                                          const int* q, float tau, int iso,
+                                          const float* __restrict__ w, float* H, float* Lo) {
   const float s = 0.70710677f;  // float32(1 / sqrt(2))
   const float vq = tv_vp(v, t, q[0], q[1], q[2]);
   bool inside = true;
@@ -560,21 +561,24 @@ __device__ __forceinline__ void tv_coeffs(const float* __restrict__ v, const TvD
     if (!t.circ && q[a] >= t.d[a]) inside = false;
   }
   if (!inside) return;
+  const float tq =
+      w ? __fmul_rn(tau, w[(static_cast<int64_t>(q[0]) * t.d[1] + q[1]) * t.d[2] + q[2]]) : tau;
   if (iso) {
-    float w[3];
+    float hv[3];
     int K = 0;
-    for (int a = t.first; a < 3; ++a) w[K++] = H[a];
-    l21_shrink(w, K, tau);
+    for (int a = t.first; a < 3; ++a) hv[K++] = H[a];
+    l21_shrink(hv, K, tq);
     K = 0;
-    for (int a = t.first; a < 3; ++a) H[a] = w[K++];
+    for (int a = t.first; a < 3; ++a) H[a] = hv[K++];
   } else {
-    for (int a = t.first; a < 3; ++a) H[a] = l1_shrink(H[a], tau);
+    for (int a = t.first; a < 3; ++a) H[a] = l1_shrink(H[a], tq);
   }
 }
 
 __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ v,
                                                       float* __restrict__ out, TvDims t,
                                                       float tau, float invK, int iso,
+                                                      const float* __restrict__ w,
                                                       const int32_t* pred) {
   if (pred_off(pred)) return;
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
@@ -583,19 +587,121 @@ __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ 
     const int p[3] = {static_cast<int>(e / (static_cast<int64_t>(t.d[1]) * t.d[2])),
                       static_cast<int>((e / t.d[2]) % t.d[1]), static_cast<int>(e % t.d[2])};
     float Hp[3], Lp[3];
-    tv_coeffs(v, t, p, tau, iso, Hp, Lp);
+    tv_coeffs(v, t, p, tau, iso, w, Hp, Lp);
     float acc = 0.0f;
     for (int a = t.first; a < 3; ++a) {
       int q[3] = {p[0], p[1], p[2]};
       q[a] = (p[a] == 0) ? t.P[a] - 1 : p[a] - 1;
       float Hq[3], Lq[3];
-      tv_coeffs(v, t, q, tau, iso, Hq, Lq);
+      tv_coeffs(v, t, q, tau, iso, w, Hq, Lq);
       const float lo = __fmul_rn(s, __fadd_rn(Lp[a], Lq[a]));
       const float hi = __fmul_rn(s, __fsub_rn(Hq[a], Hp[a]));
       const float term = __fadd_rn(lo, hi);
       acc = (a == t.first) ? term : __fadd_rn(acc, term);
     }
     out[e] = __fmul_rn(invK, acc);
+  }
+}
+
+// ------------------------------------------------- TV prox VJP (autodiff) --
+// Cotangents of out = tv_prox(v; tau * w) with respect to v and w, for the
+// differentiable unrolled reconstruction.  Pass 1, per padded coefficient
+// location q: the cotangents of the low-pass Lo_a(q) and (through the
+// shrink Jacobian) of the high-pass H_a(q), plus dw(q) for q inside;
+// pass 2, per pixel p: dv(p) gathered over the padded locations that
+// replicate p.  A cotangent gout(x) is zero outside [0, d).
+
+__device__ __forceinline__ float tv_gout(const float* __restrict__ g, const TvDims& t, int q0,
+                                         int q1, int q2) {
+  if (q0 >= t.d[0] || q1 >= t.d[1] || q2 >= t.d[2]) return 0.0f;
+  return g[(static_cast<int64_t>(q0) * t.d[1] + q1) * t.d[2] + q2];
+}
+
+__global__ __launch_bounds__(256) void tv_vjp_coef_kernel(
+    const float* __restrict__ v, const float* __restrict__ gout, const float* __restrict__ w,
+    TvDims t, float tau, float invK, int iso, float* __restrict__ HB, float* __restrict__ LB,
+    float* __restrict__ gw) {
+  const int64_t np = static_cast<int64_t>(t.P[0]) * t.P[1] * t.P[2];
+  const float s = 0.70710677f;
+  const float c = s * invK;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < np; e += 256ll * gridDim.x) {
+    const int q[3] = {static_cast<int>(e / (static_cast<int64_t>(t.P[1]) * t.P[2])),
+                      static_cast<int>((e / t.P[2]) % t.P[1]), static_cast<int>(e % t.P[2])};
+    const float vq = tv_vp(v, t, q[0], q[1], q[2]);
+    const float g0 = tv_gout(gout, t, q[0], q[1], q[2]);
+    float h[3] = {0.f, 0.f, 0.f}, hb[3] = {0.f, 0.f, 0.f};
+    bool inside = true;
+    for (int a = t.first; a < 3; ++a) {
+      int qn[3] = {q[0], q[1], q[2]};
+      qn[a] = (q[a] + 1 == t.P[a]) ? 0 : q[a] + 1;
+      const float vn = tv_vp(v, t, qn[0], qn[1], qn[2]);
+      h[a] = s * (vn - vq);
+      const float gn = tv_gout(gout, t, qn[0], qn[1], qn[2]);
+      LB[(a - t.first) * np + e] = c * (g0 + gn);
+      hb[a] = c * (gn - g0);  // cotangent of the shrunk coefficient
+      if (!t.circ && q[a] >= t.d[a]) inside = false;
+    }
+    float dtau = 0.0f;
+    if (inside) {
+      const float tq = w ? tau * w[(static_cast<int64_t>(q[0]) * t.d[1] + q[1]) * t.d[2] + q[2]]
+                         : tau;
+      if (iso) {
+        float r2 = 0.0f;
+        for (int a = t.first; a < 3; ++a) r2 += h[a] * h[a];
+        const float r = sqrtf(r2);
+        if (r > tq) {
+          float hdot = 0.0f;
+          for (int a = t.first; a < 3; ++a) hdot += h[a] * hb[a];
+          const float k1 = 1.0f - tq / r, k2 = tq / (r * r * r);
+          for (int a = t.first; a < 3; ++a) hb[a] = k1 * hb[a] + k2 * h[a] * hdot;
+          dtau = -hdot / r;
+        } else {
+          for (int a = t.first; a < 3; ++a) hb[a] = 0.0f;
+        }
+      } else {
+        for (int a = t.first; a < 3; ++a) {
+          if (fabsf(h[a]) > tq) {
+            dtau -= (h[a] > 0.0f ? hb[a] : -hb[a]);
+          } else {
+            hb[a] = 0.0f;
+          }
+        }
+      }
+      if (gw) gw[(static_cast<int64_t>(q[0]) * t.d[1] + q[1]) * t.d[2] + q[2]] = tau * dtau;
+    }
+    for (int a = t.first; a < 3; ++a) HB[(a - t.first) * np + e] = hb[a];
+  }
+}
+
+__global__ __launch_bounds__(256) void tv_vjp_v_kernel(const float* __restrict__ HB,
+                                                       const float* __restrict__ LB, TvDims t,
+                                                       float* __restrict__ gv) {
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  const int64_t np = static_cast<int64_t>(t.P[0]) * t.P[1] * t.P[2];
+  const float s = 0.70710677f;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+    const int p[3] = {static_cast<int>(e / (static_cast<int64_t>(t.d[1]) * t.d[2])),
+                      static_cast<int>((e / t.d[2]) % t.d[1]), static_cast<int>(e % t.d[2])};
+    // padded locations replicating p: q_a in {p_a} or {p_a, d_a} on the last index
+    int nalt[3];
+    for (int a = 0; a < 3; ++a) nalt[a] = (!t.circ && a >= t.first && p[a] == t.d[a] - 1) ? 2 : 1;
+    float acc = 0.0f;
+    for (int i0 = 0; i0 < nalt[0]; ++i0)
+      for (int i1 = 0; i1 < nalt[1]; ++i1)
+        for (int i2 = 0; i2 < nalt[2]; ++i2) {
+          const int q[3] = {p[0] + i0, p[1] + i1, p[2] + i2};
+          const int64_t eq = (static_cast<int64_t>(q[0]) * t.P[1] + q[1]) * t.P[2] + q[2];
+          for (int a = t.first; a < 3; ++a) {
+            const int64_t o = (a - t.first) * np;
+            int qm[3] = {q[0], q[1], q[2]};
+            qm[a] = (q[a] == 0) ? t.P[a] - 1 : q[a] - 1;
+            const int64_t em = (static_cast<int64_t>(qm[0]) * t.P[1] + qm[1]) * t.P[2] + qm[2];
+            // vp(q) enters Lo_a(q), H_a(q) as the left value and
+            // Lo_a(q - e_a), H_a(q - e_a) as the right value
+            acc += s * (LB[o + eq] - HB[o + eq]) + s * (LB[o + em] + HB[o + em]);
+          }
+        }
+    gv[e] = acc;
   }
 }
 
@@ -858,8 +964,52 @@ int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out,
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
   hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                      as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
-                     isotropic, pred);
+                     isotropic, nullptr, pred);
   SCXR_LAUNCHED("tv prox");
+  return SCXR_OK;
+}
+
+int scxr_tv_prox_w(int32_t ndim, const int32_t* shape, const float* v, const float* w,
+                   float* out, float tau, int32_t isotropic, int32_t circular,
+                   const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(v && out && v != out, SCXR_ERR_ARG,
+               "bad tv_prox_w arguments (out must not alias v)");
+  solver::TvDims t;
+  const int rc = tv_dims(ndim, shape, circular, t);
+  if (rc) return rc;
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
+                     isotropic, w, pred);
+  SCXR_LAUNCHED("tv prox w");
+  return SCXR_OK;
+}
+
+int64_t scxr_tv_prox_vjp_ws(int32_t ndim, const int32_t* shape, int32_t circular) {
+  solver::TvDims t;
+  if (tv_dims(ndim, shape, circular, t) != SCXR_OK) return -1;
+  return 2LL * ndim * t.P[0] * t.P[1] * t.P[2];
+}
+
+int scxr_tv_prox_vjp(int32_t ndim, const int32_t* shape, const float* v, const float* w,
+                     const float* gout, float tau, int32_t isotropic, int32_t circular,
+                     float* gv, float* gw, float* ws, void* stream) {
+  SCXR_REQUIRE(v && gout && gv && ws, SCXR_ERR_ARG, "bad tv_prox_vjp arguments");
+  SCXR_REQUIRE(!gw || w, SCXR_ERR_ARG, "tv_prox_vjp: gw needs the weight map w");
+  solver::TvDims t;
+  const int rc = tv_dims(ndim, shape, circular, t);
+  if (rc) return rc;
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  const int64_t np = static_cast<int64_t>(t.P[0]) * t.P[1] * t.P[2];
+  float* HB = ws;
+  float* LB = ws + ndim * np;
+  const float invK = 1.0f / static_cast<float>(2 * ndim);
+  hipLaunchKernelGGL(solver::tv_vjp_coef_kernel, dim3(solver::elem_grid(np)), dim3(256), 0,
+                     as_stream(stream), v, gout, w, t, tau, invK, isotropic, HB, LB, gw);
+  SCXR_LAUNCHED("tv prox vjp coef");
+  hipLaunchKernelGGL(solver::tv_vjp_v_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), HB, LB, t, gv);
+  SCXR_LAUNCHED("tv prox vjp v");
   return SCXR_OK;
 }
 

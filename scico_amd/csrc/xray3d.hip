@@ -40,7 +40,7 @@ namespace xr3d {
 
 enum { MODE_GENERAL = 0, MODE_SLICE = 1 };
 
-constexpr int FB = 16;  // forward: detector bins per wave
+constexpr int FB = 8;   // forward: detector bins per wave
 constexpr int AR = 8;   // adjoint: pixels per wave (2 x 4 block)
 
 struct Plan {
@@ -49,10 +49,11 @@ struct Plan {
   int sax = -1, dax = -1, delta = 0, pax = -1, qax = -1;
   int P = 0, Q = 0, S = 0, SQ = 0, U = 0;  // SQ = slice quads (slices padded to 4)
   int cpr = 32;                            // forward candidates per row
+  int spg = 2;                             // forward slots per bin group
   bool vol_direct = false, proj_direct = false;
   float* mats = nullptr;   // (V, 2, 4) float32
   float4* geom = nullptr;  // slice mode: (a, b, t, 0) in-plane row per view
-  float* volc = nullptr;   // canonical volume workspace (P Q 4 SQ)
+  float* volc = nullptr;   // canonical volume workspace (P Q + 1 rows of 4 SQ; last row 0)
   float* proja = nullptr;  // canonical projection workspace (V U 4 SQ)
 };
 
@@ -118,6 +119,17 @@ __device__ __forceinline__ float4 fma4(float w, float4 x, float4 acc) {
 }
 
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Raw buffer access: the row offset is wave-uniform (SGPR soffset), the
+// lane offset a constant VGPR, so a row load costs no address arithmetic.
+// (Buffer word 3 for gfx9-family raw buffers.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0), bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
 __device__ __forceinline__ float rlf(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
@@ -127,10 +139,13 @@ __device__ __forceinline__ float rlf(float v, int lane) {
 // Forward: each wave owns FB consecutive bins of one view for 256 slices
 // and gathers the strip of pixels that touch them, row by row along the
 // slow in-plane axis.  Lane kk of a row pass holds candidate kk of the
-// row, ordered so that its bin is non-decreasing; the wave walks the
-// candidates group by group (bin ul = u0 - 1 .. u0 + FB - 1), so every
-// accumulator index is static.
-template <int CPR>
+// row, ordered so that its bin is non-decreasing; ballots give the first
+// candidate of every bin group g (ul = u0 + g, g = -1 .. FB - 1).  Each
+// group has SPG static slots (a group holds at most SPG pixels: pixel
+// centres are at least |f| apart along the fast axis), so the whole row is
+// straight-line code: every slot's row of slices is loaded independently
+// and accumulated into static registers, the empty slots skipped.
+template <int CPR, int SPG>
 __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict__ vol,
                                                         float4* __restrict__ proj,
                                                         const float4* __restrict__ geom, int P,
@@ -140,17 +155,24 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
   constexpr int RPP = 64 / CPR;
   constexpr int BIG = INT_MAX / 2;
   const int lane = threadIdx.x & 63;
-  const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int v = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (v >= V) return;
   const int u0 = blockIdx.x * FB;
-  const int quad = blockIdx.z * 64 + lane;
-  const bool qok = quad < SQ;
-  const int qld = qok ? quad : 0;
+  const unsigned quad = blockIdx.z * 64 + lane;
+  const bool qok = quad < static_cast<unsigned>(SQ);
+  const unsigned qoff = (qok ? quad : 0u) * 16u;
+  // vol holds P Q rows of SQ slice quads plus one zero row (read by empty
+  // slots)
+  const int rowb = SQ * 16;
+  const int zoff = P * Q * rowb;
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(vol, zoff + rowb);
   const float4 g = geom[v];
   const float a = g.x, b = g.y, t = g.z;
   const bool qfast = fabsf(b) >= fabsf(a);
   const float f = qfast ? b : a, sc = qfast ? a : b;
   const int nslow = qfast ? P : Q, nfast = qfast ? Q : P;
+  // pixel index step along the fast axis, in candidate order
+  const int kstep = (f > 0.0f ? 1 : -1) * (qfast ? 1 : Q);
   const float plo = static_cast<float>(u0) - 0.75f;
   const float phi = static_cast<float>(u0 + FB) + 0.25f;
   const float rf = 1.0f / f;
@@ -168,7 +190,9 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
     const float ka = (plo - c) * rf - 0.5f, kb = (phi - c) * rf - 0.5f;
     const int klo = max(0, static_cast<int>(floorf(fminf(ka, kb))) - 1);
     const int khi = min(nfast - 1, static_cast<int>(ceilf(fmaxf(ka, kb))) + 1);
-    int gi = BIG, pix = 0;
+    const int k0 = (f > 0.0f) ? klo : khi;  // fast index of candidate 0
+    const int pix0 = qfast ? row * Q + k0 : k0 * Q + row;
+    int gi = BIG;
     float wl = 0.0f, wr = 0.0f;
     if (row < nslow && kk <= khi - klo) {
       const int k = (f > 0.0f) ? klo + kk : khi - kk;
@@ -177,22 +201,40 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
       gi = tp.ul - u0;
       wl = tp.wl;
       wr = tp.wr;
-      pix = p * Q + q;
     }
 #pragma unroll
     for (int rr = 0; rr < RPP; ++rr) {
-      int idx = rr * CPR;
-      const int end = idx + CPR;
-      while (idx < end && rl(gi, idx) < -1) ++idx;
+      if (r0 + rr >= nslow) break;
+      const int base = rr * CPR;
+      const int rpix = rl(pix0, base);
+      int first[FB + 2];
+#pragma unroll
+      for (int gg = -1; gg <= FB; ++gg) {
+        const unsigned long long bal = __ballot(gi < gg);
+        int cnt;
+        if (CPR == 64) {
+          cnt = __popcll(bal);
+        } else {
+          const unsigned part = static_cast<unsigned>(bal >> (rr * CPR)) &
+                                (CPR == 32 ? 0xffffffffu : ((1u << CPR) - 1u));
+          cnt = __popc(part);
+        }
+        first[gg + 1] = base + cnt;
+      }
 #pragma unroll
       for (int gg = -1; gg < FB; ++gg) {
-        while (idx < end && rl(gi, idx) == gg) {
-          const float cl = rlf(wl, idx), cr = rlf(wr, idx);
-          const size_t o = static_cast<size_t>(rl(pix, idx)) * SQ;
-          const float4 x = vol[o + qld];
+#pragma unroll
+        for (int k = 0; k < SPG; ++k) {
+          // Empty slots read the zero row with the weights of lane
+          // base + CPR - 1, which never holds a candidate (zero weights).
+          const int L = first[gg + 1] + k;
+          const bool ok = L < first[gg + 2];
+          const int Lc = ok ? L : base + CPR - 1;
+          const int soff = ok ? (rpix + (L - base) * kstep) * rowb : zoff;
+          const float4 x = bload4(vr, qoff, soff);
+          const float cl = rlf(wl, Lc), cr = rlf(wr, Lc);
           if (gg >= 0) acc[gg] = fma4(cl, x, acc[gg]);
           if (gg + 1 < FB) acc[gg + 1] = fma4(cr, x, acc[gg + 1]);
-          ++idx;
         }
       }
     }
@@ -217,7 +259,9 @@ __global__ __launch_bounds__(256) void adj_slice_kernel(const float4* __restrict
   const int p0 = blockIdx.y * 4 + 2 * (w >> 1), q0 = blockIdx.x * 8 + 4 * (w & 1);
   const int quad = blockIdx.z * 64 + lane;
   const bool qok = quad < SQ;
-  const int qld = qok ? quad : 0;
+  const unsigned qoff = (qok ? quad : 0) * 16u;
+  const int rowb = SQ * 16;
+  const __amdgpu_buffer_rsrc_t yr = make_rsrc(proj, V * U * rowb);
   const int r = lane & (AR - 1), jv = lane / AR;
   const int pp = p0 + (r >> 2), qq = q0 + (r & 3);
   const bool pok = pp < P && qq < Q;
@@ -239,15 +283,15 @@ __global__ __launch_bounds__(256) void adj_slice_kernel(const float4* __restrict
     }
     const int nv = min(64 / AR, V - v0);
     for (int j = 0; j < nv; ++j) {
-      const size_t vb = static_cast<size_t>(v0 + j) * U;
+      const int vb = (v0 + j) * U;
 #pragma unroll
       for (int i = 0; i < AR; ++i) {
         const int L = j * AR + i;
         const int u = rl(ul, L);
         const float cl = rlf(wl, L), cr = rlf(wr, L);
         const int lo = min(max(u, 0), U - 1), hi = min(max(u + 1, 0), U - 1);
-        const float4 ylo = proj[(vb + lo) * SQ + qld];
-        const float4 yhi = proj[(vb + hi) * SQ + qld];
+        const float4 ylo = bload4(yr, qoff, (vb + lo) * rowb);
+        const float4 yhi = bload4(yr, qoff, (vb + hi) * rowb);
         acc[i] = fma4(cr, yhi, fma4(cl, ylo, acc[i]));
       }
     }
@@ -451,7 +495,11 @@ static bool try_slice(Plan& p, const std::vector<float>& m, int sax, int dax) {
   if (!(fmin > 0.0f)) return false;
   const double need = std::ceil((FB + 1) / static_cast<double>(fmin)) + 6.0;
   if (need > 64.0) return false;
-  p.cpr = need <= 32.0 ? 32 : 64;
+  p.cpr = need <= 16.0 ? 16 : (need <= 32.0 ? 32 : 64);
+  // pixels of one bin group (left in [g, g + 1)) along the fast axis
+  const double spg = std::ceil(1.0 / static_cast<double>(fmin) + 0.01);
+  if (spg > 3.0) return false;
+  p.spg = spg <= 2.0 ? 2 : 3;
   p.mode = MODE_SLICE;
   p.sax = sax;
   p.dax = dax;
@@ -463,7 +511,13 @@ static bool try_slice(Plan& p, const std::vector<float>& m, int sax, int dax) {
   p.S = S;
   p.SQ = (S + 3) / 4;
   p.U = e == 0 ? p.D0 : p.D1;
-  p.vol_direct = sax == 2 && S % 4 == 0;
+  // The forward reads the canonical volume with one extra zero row, so the
+  // volume is always staged; buffer offsets are 32-bit.
+  p.vol_direct = false;
+  const double rowb = 16.0 * ((S + 3) / 4);
+  if ((static_cast<double>(p.P) * p.Q + 1) * rowb >= 2147483647.0) return false;
+  const int Ue = e == 0 ? p.D0 : p.D1;
+  if (static_cast<double>(p.V) * Ue * rowb >= 2147483647.0) return false;
   p.proj_direct = dax == 1 && delta == 0 && p.D1 == S && S % 4 == 0;
   return true;
 }
@@ -514,10 +568,11 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
     if (rc) return rc;
     SCXR_HIP(hipMemcpy(p.geom, g.data(), V * sizeof(float4), hipMemcpyHostToDevice));
     const size_t s4 = 4 * static_cast<size_t>(p.SQ);
-    if (!p.vol_direct) {
-      rc = dev_alloc(reinterpret_cast<void**>(&p.volc),
-                     static_cast<size_t>(p.P) * p.Q * s4 * sizeof(float));
+    {
+      const size_t rows = static_cast<size_t>(p.P) * p.Q;
+      rc = dev_alloc(reinterpret_cast<void**>(&p.volc), (rows + 1) * s4 * sizeof(float));
       if (rc) return rc;
+      SCXR_HIP(hipMemset(p.volc + rows * s4, 0, s4 * sizeof(float)));
     }
     if (!p.proj_direct) {
       rc = dev_alloc(reinterpret_cast<void**>(&p.proja),
@@ -630,14 +685,27 @@ int forward(const Plan& p, const float* vol, float* proj, const int32_t* pred, h
   }
   float* yc = p.proj_direct ? proj : p.proja;
   dim3 grid(ceil_div(p.U, FB), ceil_div(p.V, 4), ceil_div(p.SQ, 64));
-  if (p.cpr == 32)
-    hipLaunchKernelGGL(fwd_slice_kernel<32>, grid, dim3(256), 0, s,
-                       reinterpret_cast<const float4*>(vc), reinterpret_cast<float4*>(yc), p.geom,
-                       p.P, p.Q, p.SQ, p.U, p.V, pred);
-  else
-    hipLaunchKernelGGL(fwd_slice_kernel<64>, grid, dim3(256), 0, s,
-                       reinterpret_cast<const float4*>(vc), reinterpret_cast<float4*>(yc), p.geom,
-                       p.P, p.Q, p.SQ, p.U, p.V, pred);
+  const float4* v4 = reinterpret_cast<const float4*>(vc);
+  float4* y4 = reinterpret_cast<float4*>(yc);
+#define SCXR_FWD3(C, G)                                                                       \
+  hipLaunchKernelGGL((fwd_slice_kernel<C, G>), grid, dim3(256), 0, s, v4, y4, p.geom, p.P, p.Q, \
+                     p.SQ, p.U, p.V, pred)
+  if (p.spg == 2) {
+    if (p.cpr == 16)
+      SCXR_FWD3(16, 2);
+    else if (p.cpr == 32)
+      SCXR_FWD3(32, 2);
+    else
+      SCXR_FWD3(64, 2);
+  } else {
+    if (p.cpr == 16)
+      SCXR_FWD3(16, 3);
+    else if (p.cpr == 32)
+      SCXR_FWD3(32, 3);
+    else
+      SCXR_FWD3(64, 3);
+  }
+#undef SCXR_FWD3
   SCXR_LAUNCHED("xr3d slice forward");
   if (!p.proj_direct) return canon_to_proj(p, yc, proj, pred, s);
   return SCXR_OK;

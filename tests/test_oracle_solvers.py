@@ -155,3 +155,40 @@ def test_apgm_fixed_point():
     prox = lambda v, s: S.l1_prox(v, lam * s)
     x, res = S.apgm(grad, prox, 1.0, np.zeros_like(y), 50)
     np.testing.assert_allclose(x, S.l1_prox(y, lam), atol=1e-6)
+
+
+@pytest.mark.parametrize("shape", [(9, 7), (5, 6, 4), (11,)])
+@pytest.mark.parametrize("iso", [True, False])
+@pytest.mark.parametrize("circular", [False, True])
+def test_tv_prox_weighted_pins(shape, iso, circular):
+    """The weighted prox is the reference prox at w = 1 (float32, bitwise) and
+    a constant weight c acts as lam * c (float64)."""
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(shape).astype(np.float32)
+    ref = S.tv_prox(v, 0.13, iso, circular)
+    np.testing.assert_array_equal(S.tv_prox_weighted(v, 0.13, None, iso, circular), ref)
+    np.testing.assert_array_equal(
+        S.tv_prox_weighted(v, 0.13, np.ones(shape, np.float32), iso, circular), ref)
+    a = S.tv_prox_weighted(v, 0.13, np.full(shape, 1.7), iso, circular, dtype=np.float64)
+    b = S.tv_prox_weighted(v, 0.13 * 1.7, None, iso, circular, dtype=np.float64)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+    # the float64 twin agrees with the float32 restatement to float32 rounding
+    c = S.tv_prox_weighted(v, 0.13, None, iso, circular, dtype=np.float64)
+    np.testing.assert_allclose(c, ref, rtol=1e-5, atol=1e-6)
+
+
+def test_unrolled_apgm_oracle_matches_apgm():
+    """unrolled_apgm_tv at w = 1 is apgm with the reference TV prox (the two
+    restatements round the momentum and step scalars differently)."""
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((40, 64)).astype(np.float32) / 8
+    AT = A.T.copy()
+    y = rng.standard_normal(40).astype(np.float32)
+    fA = lambda v: (A @ v.ravel()).astype(np.float32)
+    fAT = lambda r: (AT @ r).reshape(8, 8).astype(np.float32)
+    L = float(np.linalg.norm(A.astype(np.float64), 2) ** 2) * 1.01
+    lam = 0.05
+    x = S.unrolled_apgm_tv(y, fA, fAT, L, lam, None, 12, np.zeros((8, 8), np.float32))
+    grad = lambda v: fAT((fA(v) - y).astype(np.float32))
+    xr, _ = S.apgm(grad, lambda v, s: S.tv_prox(v, lam * s), L, np.zeros((8, 8), np.float32), 12)
+    np.testing.assert_allclose(x, xr, rtol=1e-4, atol=1e-5)
