@@ -291,6 +291,68 @@ int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int3
 int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,
               int32_t inverse, const int32_t* pred, void* stream);
 
+/* ------------------------------------------- ragged batches of 2D slices --
+ * Many independent XRayTransform2D problems (different M, N, A, D) in
+ * single launches.  The reference batches same-shape images with
+ * jax.vmap / pmap (scico/flax/examples/data_generation.py:153-190); ragged
+ * batches have no reference counterpart.  Slice s's image occupies
+ * elements [ioff[s], ioff[s] + M N) of packed image vectors, its sinogram
+ * [soff[s], soff[s] + A D) of packed sinograms, and its FD stacks (2, M, N)
+ * start at 2 ioff[s].  Per-slice results are bitwise those of the
+ * single-slice entry points on the same slice.
+ *   dims: host, ns x 4 (M, N, A, D); angles: host, concatenated (sum A);
+ *   geo: host, ns x 5 (x0[0], x0[1], dx[0], dx[1], y0) as in
+ *   scxr_xr2d_plan_create; vpg: views per forward group (1, 2, 4).
+ * pred (nullable) holds one device predicate per slice at pred[s * pstride]
+ * (a slice whose predicate is 0 is skipped). */
+typedef struct scxr_xr2d_batch scxr_xr2d_batch;
+
+int scxr_xr2d_batch_create(int32_t ns, const int32_t* dims, const float* angles,
+                           const double* geo, int32_t vpg, scxr_xr2d_batch** out);
+int scxr_xr2d_batch_destroy(scxr_xr2d_batch* plan);
+/* out (host, 11): ns, image elements, sinogram elements, forward items,
+ * adjoint items, transpose items, CG/ADMM blocks, max D, vpg, adjoint
+ * tiles (dot partials), forward groups (residual partials). */
+int scxr_xr2d_batch_info(const scxr_xr2d_batch* plan, int64_t* out);
+int scxr_xr2d_batch_offsets(const scxr_xr2d_batch* plan, int64_t* ioff /* host, ns */,
+                            int64_t* soff /* host, ns */);
+/* Persistent grid sizes (0 = one workgroup per item) and item order
+ * (0 slice-major, 1 round robin, 2 decreasing cost); set by the autotuner. */
+int scxr_xr2d_batch_configure(scxr_xr2d_batch* plan, int32_t fwd_grid, int32_t adj_grid,
+                              int32_t order);
+/* sino = A im for every slice; with y and res_ws also per-group partials of
+ * ||y - A im||^2 (res_ws has "forward groups" doubles). */
+int scxr_xr2d_batch_forward(const scxr_xr2d_batch* plan, const float* im, float* sino,
+                            const float* y, double* res_ws, const int32_t* pred,
+                            int32_t pstride, void* stream);
+/* im = A^T sino with the epilogues of scxr_xr2d_adjoint_epi per slice
+ * (dot_ws has "adjoint tiles" doubles). */
+int scxr_xr2d_batch_adjoint(const scxr_xr2d_batch* plan, const float* sino, float* im,
+                            int32_t kind, float rho, int32_t circular, const float* p,
+                            const float* z, const float* u, double* dot_ws, const int32_t* pred,
+                            int32_t pstride, void* stream);
+/* Per-slice CG and ADMM passes (scxr_cg_start / scxr_cg_iter / scxr_admm_zu2d
+ * / scxr_admm_dual2d / scxr_admm_itstat on every slice):
+ *   state: ns x (8 + 2 (maxiter + 1)) doubles, flags: ns x (maxiter + 1),
+ *   ws, ws_prml, ws_g, dual: 2 x (CG/ADMM blocks) doubles,
+ *   res: forward-group partials, pap_parts: adjoint-tile partials,
+ *   hist: ns x hist_cap x 5, counters: ns int32. */
+int scxr_batch_cg_start(const scxr_xr2d_batch* plan, const float* b, const float* Ax, float* r,
+                        float* p, double* state, int32_t* flags, int32_t maxiter, double tol,
+                        double atol, double* ws, void* stream);
+int scxr_batch_cg_iter(const scxr_xr2d_batch* plan, int32_t k, int32_t maxiter, float* x,
+                       float* r, float* p, const float* Ap, double* state, int32_t* flags,
+                       const double* pap_parts, double* ws, void* stream);
+int scxr_batch_admm_zu(const scxr_xr2d_batch* plan, const float* x, float* z, float* u,
+                       float* zold, float thr, int32_t circular, int32_t isotropic,
+                       double* ws_prml, double* ws_g, void* stream);
+int scxr_batch_admm_dual(const scxr_xr2d_batch* plan, const float* z, const float* zold,
+                         float rho, int32_t circular, double* ws, void* stream);
+int scxr_batch_admm_itstat(const scxr_xr2d_batch* plan, const double* res, const double* g,
+                           const double* prml, const double* dual, const double* cg_state,
+                           int32_t maxiter, double lam, double rho, double* hist,
+                           int32_t* counters, int32_t hist_cap, void* stream);
+
 /* ------------------------------------------------ 3D parallel-beam X-ray --
  * Replaces scico.linop.xray.XRayTransform3D:
  *   constructor            scico/linop/xray/_xray3d.py:54-93

@@ -24,6 +24,7 @@ SOURCES = [
     "common.cpp",
     "xray2d.hip",
     "xray3d.hip",
+    "batch2d.hip",
     "vec.hip",
     "solver.hip",
     "fft.hip",

@@ -62,6 +62,32 @@ SIGNATURES = [
     ("scxr_xr2d_adjoint_epi", c_int32,
      [c_void_p, c_void_p, c_void_p, c_int32, c_float, c_int32, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p]),
+    # ragged batches of 2D slices
+    ("scxr_xr2d_batch_create", c_int32,
+     [c_int32, c_int32_p, c_float_p, c_double_p, c_int32, POINTER(c_void_p)]),
+    ("scxr_xr2d_batch_destroy", c_int32, [c_void_p]),
+    ("scxr_xr2d_batch_info", c_int32, [c_void_p, POINTER(c_int64)]),
+    ("scxr_xr2d_batch_offsets", c_int32, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
+    ("scxr_xr2d_batch_configure", c_int32, [c_void_p, c_int32, c_int32, c_int32]),
+    ("scxr_xr2d_batch_forward", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    ("scxr_xr2d_batch_adjoint", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_int32, c_float, c_int32, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_int32, c_void_p]),
+    ("scxr_batch_cg_start", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double,
+      c_double, c_void_p, c_void_p]),
+    ("scxr_batch_cg_iter", c_int32,
+     [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p]),
+    ("scxr_batch_admm_zu", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_void_p,
+      c_void_p, c_void_p]),
+    ("scxr_batch_admm_dual", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
+    ("scxr_batch_admm_itstat", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double, c_double,
+      c_void_p, c_void_p, c_int32, c_void_p]),
     # 3D projector
     ("scxr_xr3d_plan_create", c_int32,
      [c_int32, c_int32, c_int32, c_int32, c_float_p, c_int32, c_int32, c_int32,

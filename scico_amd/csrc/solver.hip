@@ -156,14 +156,15 @@ __global__ __launch_bounds__(256) void l1_prox_kernel(int64_t n, const float* __
 // ------------------------------------------------------------------ CG --
 
 // r = b - A x0; p = r; partials of b^T b and r^T r.
-__global__ __launch_bounds__(256) void cg_start_kernel(int64_t n, const float* __restrict__ b,
-                                                       const float* __restrict__ Ax,
-                                                       float* __restrict__ r,
-                                                       float* __restrict__ p,
-                                                       double* __restrict__ ws) {
+// Kernel bodies take their (virtual) block index vb and grid size vg, so
+// that the batched kernels run exactly the same per-solve arithmetic.
+__device__ __forceinline__ void cg_start_body(int64_t n, const float* __restrict__ b,
+                                              const float* __restrict__ Ax,
+                                              float* __restrict__ r, float* __restrict__ p,
+                                              double* __restrict__ ws, int vb, int vg) {
   __shared__ double sh[4];
   double bb = 0.0, rr = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+  for (int64_t e = vb * 256ll + threadIdx.x; e < n; e += 256ll * vg) {
     const float be = b[e];
     const float re = __fsub_rn(be, Ax[e]);
     r[e] = re;
@@ -172,15 +173,23 @@ __global__ __launch_bounds__(256) void cg_start_kernel(int64_t n, const float* _
     rr += static_cast<double>(re) * re;
   }
   bb = block_sum_f64(bb, sh);
-  if (threadIdx.x == 0) ws[blockIdx.x] = bb;
+  if (threadIdx.x == 0) ws[vb] = bb;
   rr = block_sum_f64(rr, sh);
-  if (threadIdx.x == 0) ws[gridDim.x + blockIdx.x] = rr;
+  if (threadIdx.x == 0) ws[vg + vb] = rr;
 }
 
-__global__ __launch_bounds__(256) void cg_start_fin(int nparts, const double* __restrict__ ws,
-                                                    double* __restrict__ state,
-                                                    int32_t* __restrict__ flags, int maxiter,
-                                                    double tol, double atol) {
+__global__ __launch_bounds__(256) void cg_start_kernel(int64_t n, const float* __restrict__ b,
+                                                       const float* __restrict__ Ax,
+                                                       float* __restrict__ r,
+                                                       float* __restrict__ p,
+                                                       double* __restrict__ ws) {
+  cg_start_body(n, b, Ax, r, p, ws, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void cg_start_fin_body(int nparts, const double* __restrict__ ws,
+                                                  double* __restrict__ state,
+                                                  int32_t* __restrict__ flags, int maxiter,
+                                                  double tol, double atol) {
   __shared__ double sh[4];
   const double bb = all_sum_f64(ws, nparts, sh);
   const double rr = all_sum_f64(ws + nparts, nparts, sh);
@@ -197,22 +206,28 @@ __global__ __launch_bounds__(256) void cg_start_fin(int nparts, const double* __
   }
 }
 
-// alpha = num_k / p^T A p; x += alpha p; r -= alpha A p; partials of r^T r.
-__global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, int k, int maxiter,
-                                                    float* __restrict__ x, float* __restrict__ r,
-                                                    const float* __restrict__ p,
-                                                    const float* __restrict__ Ap,
+__global__ __launch_bounds__(256) void cg_start_fin(int nparts, const double* __restrict__ ws,
                                                     double* __restrict__ state,
-                                                    const int32_t* __restrict__ flags,
-                                                    const double* __restrict__ pap_parts,
-                                                    int pap_n, double* __restrict__ ws_rr) {
+                                                    int32_t* __restrict__ flags, int maxiter,
+                                                    double tol, double atol) {
+  cg_start_fin_body(nparts, ws, state, flags, maxiter, tol, atol);
+}
+
+// alpha = num_k / p^T A p; x += alpha p; r -= alpha A p; partials of r^T r.
+__device__ __forceinline__ void cg_xr_body(int64_t n, int k, int maxiter, float* __restrict__ x,
+                                           float* __restrict__ r, const float* __restrict__ p,
+                                           const float* __restrict__ Ap,
+                                           double* __restrict__ state,
+                                           const int32_t* __restrict__ flags,
+                                           const double* __restrict__ pap_parts, int pap_n,
+                                           double* __restrict__ ws_rr, int vb, int vg) {
   if (flags[k] == 0) return;
   __shared__ double sh[4];
   const double pap = all_sum_f64(pap_parts, pap_n, sh);
   const float alpha = static_cast<float>(static_cast<float>(state[ST_NUM + k]) /
                                          static_cast<float>(pap));
   double rr = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+  for (int64_t e = vb * 256ll + threadIdx.x; e < n; e += 256ll * vg) {
     const float pe = p[e];
     x[e] = __fadd_rn(x[e], __fmul_rn(alpha, pe));
     const float re = __fsub_rn(r[e], __fmul_rn(alpha, Ap[e]));
@@ -221,31 +236,51 @@ __global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, int k, int maxite
   }
   rr = block_sum_f64(rr, sh);
   if (threadIdx.x == 0) {
-    ws_rr[blockIdx.x] = rr;
-    if (blockIdx.x == 0) state[ST_NUM + (maxiter + 1) + k] = pap;
+    ws_rr[vb] = rr;
+    if (vb == 0) state[ST_NUM + (maxiter + 1) + k] = pap;
   }
 }
 
+__global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, int k, int maxiter,
+                                                    float* __restrict__ x, float* __restrict__ r,
+                                                    const float* __restrict__ p,
+                                                    const float* __restrict__ Ap,
+                                                    double* __restrict__ state,
+                                                    const int32_t* __restrict__ flags,
+                                                    const double* __restrict__ pap_parts,
+                                                    int pap_n, double* __restrict__ ws_rr) {
+  cg_xr_body(n, k, maxiter, x, r, p, Ap, state, flags, pap_parts, pap_n, ws_rr, blockIdx.x,
+             gridDim.x);
+}
+
 // beta = num_{k+1} / num_k; p = r + beta p; stopping test for k + 1.
+__device__ __forceinline__ void cg_p_body(int64_t n, int k, int maxiter,
+                                          const float* __restrict__ r, float* __restrict__ p,
+                                          double* __restrict__ state, int32_t* __restrict__ flags,
+                                          const double* __restrict__ ws_rr, int rr_n, int vb,
+                                          int vg) {
+  if (flags[k] == 0) return;
+  __shared__ double sh[4];
+  const double rr = all_sum_f64(ws_rr, rr_n, sh);
+  const float num_new = static_cast<float>(rr);
+  const float beta = num_new / static_cast<float>(state[ST_NUM + k]);
+  for (int64_t e = vb * 256ll + threadIdx.x; e < n; e += 256ll * vg)
+    p[e] = __fadd_rn(r[e], __fmul_rn(beta, p[e]));
+  if (vb == 0 && threadIdx.x == 0) {
+    state[ST_NUM + k + 1] = rr;
+    state[ST_NITER] = k + 1;
+    state[ST_RELRES] = sqrt(rr) / state[ST_BN];
+    flags[k + 1] = (k + 1 < maxiter && rr > state[ST_TOL2]) ? 1 : 0;
+  }
+}
+
 __global__ __launch_bounds__(256) void cg_p_kernel(int64_t n, int k, int maxiter,
                                                    const float* __restrict__ r,
                                                    float* __restrict__ p,
                                                    double* __restrict__ state,
                                                    int32_t* __restrict__ flags,
                                                    const double* __restrict__ ws_rr, int rr_n) {
-  if (flags[k] == 0) return;
-  __shared__ double sh[4];
-  const double rr = all_sum_f64(ws_rr, rr_n, sh);
-  const float num_new = static_cast<float>(rr);
-  const float beta = num_new / static_cast<float>(state[ST_NUM + k]);
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x)
-    p[e] = __fadd_rn(r[e], __fmul_rn(beta, p[e]));
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    state[ST_NUM + k + 1] = rr;
-    state[ST_NITER] = k + 1;
-    state[ST_RELRES] = sqrt(rr) / state[ST_BN];
-    flags[k + 1] = (k + 1 < maxiter && rr > state[ST_TOL2]) ? 1 : 0;
-  }
+  cg_p_body(n, k, maxiter, r, p, state, flags, ws_rr, rr_n, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- ADMM --
@@ -255,19 +290,16 @@ __global__ __launch_bounds__(256) void cg_p_kernel(int64_t n, int k, int maxiter
 //   Cx = C x; z_old = z; z = prox_g(Cx + u, lam / rho); u = (u + Cx) - z
 // (scico/optimize/_admm.py:364-378), plus per-workgroup partials of
 // ||C x - z||^2 (primal residual) and g(z) / lam.
-__global__ __launch_bounds__(256) void admm_zu2d_kernel(int M, int N,
-                                                        const float* __restrict__ x,
-                                                        float* __restrict__ z,
-                                                        float* __restrict__ u,
-                                                        float* __restrict__ zold, float thr,
-                                                        int circ, int iso,
-                                                        double* __restrict__ ws_prml,
-                                                        double* __restrict__ ws_g) {
+__device__ __forceinline__ void admm_zu2d_body(int M, int N, const float* __restrict__ x,
+                                               float* __restrict__ z, float* __restrict__ u,
+                                               float* __restrict__ zold, float thr, int circ,
+                                               int iso, double* __restrict__ ws_prml,
+                                               double* __restrict__ ws_g, int vb, int vg) {
   __shared__ double sh[4];
   const int64_t n = static_cast<int64_t>(M) * N;
   const bool cc = circ != 0;
   double prml = 0.0, gsum = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+  for (int64_t e = vb * 256ll + threadIdx.x; e < n; e += 256ll * vg) {
     const int i = static_cast<int>(e / N), j = static_cast<int>(e - static_cast<int64_t>(i) * N);
     const float xc = x[e];
     const int ip = (i + 1 < M) ? i + 1 : 0, jp = (j + 1 < N) ? j + 1 : 0;
@@ -297,27 +329,37 @@ __global__ __launch_bounds__(256) void admm_zu2d_kernel(int M, int N,
     }
   }
   prml = block_sum_f64(prml, sh);
-  if (threadIdx.x == 0) ws_prml[blockIdx.x] = prml;
+  if (threadIdx.x == 0) ws_prml[vb] = prml;
   gsum = block_sum_f64(gsum, sh);
-  if (threadIdx.x == 0) ws_g[blockIdx.x] = gsum;
+  if (threadIdx.x == 0) ws_g[vb] = gsum;
+}
+
+__global__ __launch_bounds__(256) void admm_zu2d_kernel(int M, int N,
+                                                        const float* __restrict__ x,
+                                                        float* __restrict__ z,
+                                                        float* __restrict__ u,
+                                                        float* __restrict__ zold, float thr,
+                                                        int circ, int iso,
+                                                        double* __restrict__ ws_prml,
+                                                        double* __restrict__ ws_g) {
+  admm_zu2d_body(M, N, x, z, u, zold, thr, circ, iso, ws_prml, ws_g, blockIdx.x, gridDim.x);
 }
 
 // Partials of ||rho * C^T (z - z_old)||^2 (dual residual, _admm.py:279-295).
-__global__ __launch_bounds__(256) void admm_dual2d_kernel(int M, int N,
-                                                          const float* __restrict__ z,
-                                                          const float* __restrict__ zold,
-                                                          float rho, int circ,
-                                                          double* __restrict__ ws) {
+__device__ __forceinline__ void admm_dual2d_body(int M, int N, const float* __restrict__ z,
+                                                 const float* __restrict__ zold, float rho,
+                                                 int circ, double* __restrict__ ws, int vb,
+                                                 int vg) {
   __shared__ double sh[4];
   const int64_t n = static_cast<int64_t>(M) * N;
   double s = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
+  for (int64_t e = vb * 256ll + threadIdx.x; e < n; e += 256ll * vg) {
     const int i = static_cast<int>(e / N), j = static_cast<int>(e - static_cast<int64_t>(i) * N);
     const float t = __fmul_rn(rho, fd::adj2d_diff(z, zold, M, N, i, j, circ != 0));
     s += static_cast<double>(t) * t;
   }
   s = block_sum_f64(s, sh);
-  if (threadIdx.x == 0) ws[blockIdx.x] = s;
+  if (threadIdx.x == 0) ws[vb] = s;
 }
 
 // ------------------------------------------------- nd ADMM-TV (any A) --
@@ -492,9 +534,17 @@ __global__ __launch_bounds__(256) void admm_dual_nd_kernel(int ndim, int d0, int
   if (threadIdx.x == 0) ws[blockIdx.x] = s;
 }
 
+__global__ __launch_bounds__(256) void admm_dual2d_kernel(int M, int N,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ zold,
+                                                          float rho, int circ,
+                                                          double* __restrict__ ws) {
+  admm_dual2d_body(M, N, z, zold, rho, circ, ws, blockIdx.x, gridDim.x);
+}
+
 // Iteration statistics row (Objective, Prml Rsdl, Dual Rsdl, CG It, CG Res)
 // appended at hist[5 * counter]; counter is a device int.
-__global__ __launch_bounds__(256) void admm_itstat_kernel(
+__device__ __forceinline__ void admm_itstat_body(
     const double* __restrict__ res, int nres, const double* __restrict__ g, int ng,
     const double* __restrict__ prml, int nprml, const double* __restrict__ dual, int ndual,
     const double* __restrict__ cg_state, double lam, double rho, double* __restrict__ hist,
@@ -516,6 +566,15 @@ __global__ __launch_bounds__(256) void admm_itstat_kernel(
     }
     *counter = it + 1;
   }
+}
+
+__global__ __launch_bounds__(256) void admm_itstat_kernel(
+    const double* __restrict__ res, int nres, const double* __restrict__ g, int ng,
+    const double* __restrict__ prml, int nprml, const double* __restrict__ dual, int ndual,
+    const double* __restrict__ cg_state, double lam, double rho, double* __restrict__ hist,
+    int32_t* __restrict__ counter, int hist_cap) {
+  admm_itstat_body(res, nres, g, ng, prml, nprml, dual, ndual, cg_state, lam, rho, hist,
+                   counter, hist_cap);
 }
 
 // ---------------------------------------------------------------- TV --
@@ -732,6 +791,152 @@ __global__ __launch_bounds__(256) void tv_norm_kernel(const float* __restrict__ 
   }
   acc = block_sum_f64(acc, sh);
   if (threadIdx.x == 0) ws[blockIdx.x] = acc;
+}
+
+// ------------------------------------------------------------- batched --
+
+struct SegBlock {
+  int s, vb, vg;
+  int64_t o, n;
+  int M, N;
+};
+
+__device__ __forceinline__ SegBlock seg_block(const SegTabs& t) {
+  SegBlock r;
+  r.s = t.blk_seg[blockIdx.x];
+  r.vb = blockIdx.x - t.blk_first[r.s];
+  r.vg = t.G[r.s];
+  r.o = t.off[r.s];
+  const int2 mn = t.MN[r.s];
+  r.M = mn.x;
+  r.N = mn.y;
+  r.n = static_cast<int64_t>(mn.x) * mn.y;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void cg_start_b(SegTabs t, const float* __restrict__ b,
+                                                  const float* __restrict__ Ax,
+                                                  float* __restrict__ r, float* __restrict__ p,
+                                                  double* __restrict__ ws) {
+  const SegBlock q = seg_block(t);
+  cg_start_body(q.n, b + q.o, Ax + q.o, r + q.o, p + q.o, ws + 2 * t.woff[q.s], q.vb, q.vg);
+}
+
+__global__ __launch_bounds__(256) void cg_start_fin_b(SegTabs t, const double* __restrict__ ws,
+                                                      double* __restrict__ state,
+                                                      int32_t* __restrict__ flags, int maxiter,
+                                                      double tol, double atol) {
+  const int s = blockIdx.x;
+  const int SS = ST_NUM + 2 * (maxiter + 1);
+  cg_start_fin_body(t.G[s], ws + 2 * t.woff[s], state + static_cast<int64_t>(s) * SS,
+                    flags + static_cast<int64_t>(s) * (maxiter + 1), maxiter, tol, atol);
+}
+
+__global__ __launch_bounds__(256) void cg_xr_b(SegTabs t, int k, int maxiter,
+                                               float* __restrict__ x, float* __restrict__ r,
+                                               const float* __restrict__ p,
+                                               const float* __restrict__ Ap,
+                                               double* __restrict__ state,
+                                               const int32_t* __restrict__ flags,
+                                               const double* __restrict__ pap,
+                                               double* __restrict__ ws) {
+  const SegBlock q = seg_block(t);
+  const int SS = ST_NUM + 2 * (maxiter + 1);
+  cg_xr_body(q.n, k, maxiter, x + q.o, r + q.o, p + q.o, Ap + q.o,
+             state + static_cast<int64_t>(q.s) * SS,
+             flags + static_cast<int64_t>(q.s) * (maxiter + 1), pap + t.pap_off[q.s],
+             t.pap_n[q.s], ws + t.woff[q.s], q.vb, q.vg);
+}
+
+__global__ __launch_bounds__(256) void cg_p_b(SegTabs t, int k, int maxiter,
+                                              const float* __restrict__ r, float* __restrict__ p,
+                                              double* __restrict__ state,
+                                              int32_t* __restrict__ flags,
+                                              const double* __restrict__ ws) {
+  const SegBlock q = seg_block(t);
+  const int SS = ST_NUM + 2 * (maxiter + 1);
+  cg_p_body(q.n, k, maxiter, r + q.o, p + q.o, state + static_cast<int64_t>(q.s) * SS,
+            flags + static_cast<int64_t>(q.s) * (maxiter + 1), ws + t.woff[q.s], q.vg, q.vb,
+            q.vg);
+}
+
+__global__ __launch_bounds__(256) void admm_zu_b(SegTabs t, const float* __restrict__ x,
+                                                 float* __restrict__ z, float* __restrict__ u,
+                                                 float* __restrict__ zold, float thr, int circ,
+                                                 int iso, double* __restrict__ ws_prml,
+                                                 double* __restrict__ ws_g) {
+  const SegBlock q = seg_block(t);
+  admm_zu2d_body(q.M, q.N, x + q.o, z + 2 * q.o, u + 2 * q.o, zold + 2 * q.o, thr, circ, iso,
+                 ws_prml + t.woff[q.s], ws_g + t.woff[q.s], q.vb, q.vg);
+}
+
+__global__ __launch_bounds__(256) void admm_dual_b(SegTabs t, const float* __restrict__ z,
+                                                   const float* __restrict__ zold, float rho,
+                                                   int circ, double* __restrict__ ws) {
+  const SegBlock q = seg_block(t);
+  admm_dual2d_body(q.M, q.N, z + 2 * q.o, zold + 2 * q.o, rho, circ, ws + t.woff[q.s], q.vb,
+                   q.vg);
+}
+
+__global__ __launch_bounds__(256) void admm_itstat_b(
+    SegTabs t, const double* __restrict__ res, const double* __restrict__ g,
+    const double* __restrict__ prml, const double* __restrict__ dual,
+    const double* __restrict__ cg_state, int maxiter, double lam, double rho,
+    double* __restrict__ hist, int32_t* __restrict__ counters, int hist_cap) {
+  const int s = blockIdx.x;
+  const int SS = ST_NUM + 2 * (maxiter + 1);
+  const int w = t.woff[s], G = t.G[s];
+  admm_itstat_body(res + t.res_off[s], t.res_n[s], g + w, G, prml + w, G, dual + w, G,
+                   cg_state ? cg_state + static_cast<int64_t>(s) * SS : nullptr, lam, rho,
+                   hist + static_cast<int64_t>(s) * 5 * hist_cap, counters + s, hist_cap);
+}
+
+int batch_cg_start(const SegTabs& t, const float* b, const float* Ax, float* r, float* p,
+                   double* state, int32_t* flags, int maxiter, double tol, double atol,
+                   double* ws, hipStream_t s) {
+  hipLaunchKernelGGL(cg_start_b, dim3(t.nblk), dim3(256), 0, s, t, b, Ax, r, p, ws);
+  SCXR_LAUNCHED("batch cg start");
+  hipLaunchKernelGGL(cg_start_fin_b, dim3(t.ns), dim3(256), 0, s, t, ws, state, flags, maxiter,
+                     tol, atol);
+  SCXR_LAUNCHED("batch cg start fin");
+  return SCXR_OK;
+}
+
+int batch_cg_iter(const SegTabs& t, int k, int maxiter, float* x, float* r, float* p,
+                  const float* Ap, double* state, int32_t* flags, const double* pap_parts,
+                  double* ws, hipStream_t s) {
+  hipLaunchKernelGGL(cg_xr_b, dim3(t.nblk), dim3(256), 0, s, t, k, maxiter, x, r, p, Ap, state,
+                     flags, pap_parts, ws);
+  SCXR_LAUNCHED("batch cg xr");
+  hipLaunchKernelGGL(cg_p_b, dim3(t.nblk), dim3(256), 0, s, t, k, maxiter, r, p, state, flags,
+                     ws);
+  SCXR_LAUNCHED("batch cg p");
+  return SCXR_OK;
+}
+
+int batch_admm_zu(const SegTabs& t, const float* x, float* z, float* u, float* zold, float thr,
+                  int circ, int iso, double* ws_prml, double* ws_g, hipStream_t s) {
+  hipLaunchKernelGGL(admm_zu_b, dim3(t.nblk), dim3(256), 0, s, t, x, z, u, zold, thr, circ, iso,
+                     ws_prml, ws_g);
+  SCXR_LAUNCHED("batch admm zu");
+  return SCXR_OK;
+}
+
+int batch_admm_dual(const SegTabs& t, const float* z, const float* zold, float rho, int circ,
+                    double* ws, hipStream_t s) {
+  hipLaunchKernelGGL(admm_dual_b, dim3(t.nblk), dim3(256), 0, s, t, z, zold, rho, circ, ws);
+  SCXR_LAUNCHED("batch admm dual");
+  return SCXR_OK;
+}
+
+int batch_admm_itstat(const SegTabs& t, const double* res, const double* g, const double* prml,
+                      const double* dual, const double* cg_state, int maxiter, double lam,
+                      double rho, double* hist, int32_t* counters, int hist_cap,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(admm_itstat_b, dim3(t.ns), dim3(256), 0, s, t, res, g, prml, dual, cg_state,
+                     maxiter, lam, rho, hist, counters, hist_cap);
+  SCXR_LAUNCHED("batch admm itstat");
+  return SCXR_OK;
 }
 
 unsigned elem_grid(int64_t n) {

@@ -57,16 +57,10 @@ void view_geometry(int A, const float* angles, const double* x0, const double* d
   }
 }
 
-int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx,
-           double y0, const float* angles) {
-  p.M = M;
-  p.N = N;
-  p.A = A;
-  p.D = D;
-  std::vector<float4> geom;
-  std::vector<double> rw;
-  view_geometry(A, angles, x0, dx, y0, geom, rw);
-  p.host_geom = geom;
+int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
+                 double y0, const float* angles, int vpg) {
+  view_geometry(A, angles, x0, dx, y0, h.geom, h.rw);
+  const std::vector<float4>& geom = h.geom;
   for (int a = 0; a < A; ++a)
     SCXR_REQUIRE(std::isfinite(geom[a].x) && std::isfinite(geom[a].y) &&
                      std::isfinite(geom[a].z) && geom[a].w > 0.0f && std::isfinite(geom[a].w),
@@ -76,22 +70,19 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   // Forward: views whose detector coordinate varies fastest along image
   // columns (|P1| >= |P0|) stream the image row-major; the others stream
   // its transpose, so that the 64 lanes of a wave always spread over many
-  // detector bins (few LDS atomic collisions).
+  // detector bins.
   std::vector<int> rows, cols;
   for (int a = 0; a < A; ++a)
     (std::fabs(geom[a].z) >= std::fabs(geom[a].y) ? rows : cols).push_back(a);
-  p.any_colfast = !cols.empty();
-  // Views per workgroup: enough workgroups to fill 256 CUs at least twice.
-  p.vpg = A >= 2048 ? 4 : (A >= 1024 ? 2 : 1);
-  while (p.vpg > 1 && fwd_lds_bytes(p.vpg, D) > FWD_LDS_MAX) p.vpg /= 2;
-  SCXR_REQUIRE(fwd_lds_bytes(1, D) <= FWD_LDS_MAX, SCXR_ERR_UNSUPPORTED,
+  h.any_colfast = !cols.empty();
+  SCXR_REQUIRE(fwd_lds_bytes(vpg, D) <= FWD_LDS_MAX, SCXR_ERR_UNSUPPORTED,
                "det_count %d too large for the LDS-resident forward projector", D);
-  std::vector<int> order;
-  std::vector<int4> groups;
+  h.order.clear();
+  h.groups.clear();
   for (int cls = 0; cls < 2; ++cls) {
     const std::vector<int>& list = cls ? cols : rows;
-    for (size_t s = 0; s < list.size(); s += p.vpg) {
-      const int cnt = static_cast<int>(std::min<size_t>(p.vpg, list.size() - s));
+    for (size_t s = 0; s < list.size(); s += vpg) {
+      const int cnt = static_cast<int>(std::min<size_t>(vpg, list.size() - s));
       // Lane stride along the fast axis: S * |P_fast| >= 2 (+ margin for
       // the float32 rounding of Px) keeps the lanes' footprints disjoint.
       int S = 1;
@@ -103,32 +94,53 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
       SCXR_REQUIRE(S <= FWD_SMAX, SCXR_ERR_GEOM,
                    "projected pixel spacing too small for the forward projector "
                    "(lane stride %d > %d)", S, FWD_SMAX);
-      groups.push_back(make_int4(static_cast<int>(order.size()), cnt, cls, S));
-      for (int k = 0; k < cnt; ++k) order.push_back(list[s + k]);
+      h.groups.push_back(make_int4(static_cast<int>(h.order.size()), cnt, cls, S));
+      for (int k = 0; k < cnt; ++k) h.order.push_back(list[s + k]);
     }
   }
-  p.n_groups = static_cast<int>(groups.size());
 
-  // Adjoint: detector window per (tile, view) = all bins an ADJ_TI x ADJ_TJ
-  // pixel tile can reach, plus margin for rounding.
-  // Window origin offset per view (see adj_kernel): a lower bound of
-  // floor(Px) - 1 over any tile; the window spans the projected tile
+  // Adjoint: window origin offset per view (see adj_tile): a lower bound
+  // of floor(Px) - 1 over any tile; the window spans the projected tile
   // extent plus the lower margin, the rounding of Px and bin ind + 1.
-  std::vector<float> clo(A);
+  h.clo.assign(A, 0.0f);
   int wmax = 0;
   for (int a = 0; a < A; ++a) {
     const double P0 = geom[a].y, P1 = geom[a].z;
-    clo[a] = static_cast<float>(static_cast<double>(geom[a].x) +
-                                std::min(P0, 0.0) * (ADJ_TI - 1) +
-                                std::min(P1, 0.0) * (ADJ_TJ - 1) - 1.0);
+    h.clo[a] = static_cast<float>(static_cast<double>(geom[a].x) +
+                                  std::min(P0, 0.0) * (ADJ_TI - 1) +
+                                  std::min(P1, 0.0) * (ADJ_TJ - 1) - 1.0);
     const double ext = (ADJ_TI - 1) * std::fabs(P0) + (ADJ_TJ - 1) * std::fabs(P1);
     wmax = std::max(wmax, static_cast<int>(std::ceil(ext)) + 5);
   }
-  p.adj_w = wmax;
-  SCXR_REQUIRE(p.adj_w <= ADJ_WFAST, SCXR_ERR_GEOM,
+  h.adj_w = wmax;
+  SCXR_REQUIRE(h.adj_w <= ADJ_WFAST, SCXR_ERR_GEOM,
                "projected pixel footprint too large for the adjoint tile (%d bins > %d); "
                "pixel size dx is too large relative to the detector spacing",
-               p.adj_w, ADJ_WFAST);
+               h.adj_w, ADJ_WFAST);
+  return SCXR_OK;
+}
+
+int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx,
+           double y0, const float* angles) {
+  p.M = M;
+  p.N = N;
+  p.A = A;
+  p.D = D;
+  // Views per workgroup: enough workgroups to fill 256 CUs at least twice.
+  p.vpg = A >= 2048 ? 4 : (A >= 1024 ? 2 : 1);
+  while (p.vpg > 1 && fwd_lds_bytes(p.vpg, D) > FWD_LDS_MAX) p.vpg /= 2;
+  HostTables h;
+  const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg);
+  if (rc) return rc;
+  p.host_geom = h.geom;
+  p.any_colfast = h.any_colfast;
+  p.n_groups = static_cast<int>(h.groups.size());
+  p.adj_w = h.adj_w;
+  const std::vector<float4>& geom = h.geom;
+  const std::vector<double>& rw = h.rw;
+  const std::vector<int4>& groups = h.groups;
+  const std::vector<int>& order = h.order;
+  const std::vector<float>& clo = h.clo;
 
   size_t bytes_geom = sizeof(float4) * A, bytes_rw = sizeof(double) * A;
   size_t bytes_groups = sizeof(int4) * groups.size(), bytes_order = sizeof(int) * A;
@@ -186,14 +198,14 @@ __device__ __forceinline__ Weight weight_at(float Pxmin, float P0, float P1, flo
   return Weight{fl, fminf(div_rn(dd, rw), 1.0f)};
 }
 
-// Transpose (M, N) -> (N, M) through a padded LDS tile.
-__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
-                                                        float* __restrict__ out, int M, int N,
-                                                        const int32_t* pred) {
-  if (pred_off(pred)) return;
+// Transpose (M, N) -> (N, M) through a padded LDS tile (tile row by,
+// column bx).
+__device__ __forceinline__ void transpose_tile(const float* __restrict__ in,
+                                               float* __restrict__ out, int M, int N, int by,
+                                               int bx) {
   __shared__ float tile[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+  const int i0 = by * 32, j0 = bx * 32;
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + r, j = j0 + tx;
     if (i < M && j < N) tile[r][tx] = in[static_cast<size_t>(i) * N + j];
@@ -203,6 +215,13 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
     const int j = j0 + r, i = i0 + tx;
     if (i < M && j < N) out[static_cast<size_t>(j) * M + i] = tile[tx][r];
   }
+}
+
+__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, int M, int N,
+                                                        const int32_t* pred) {
+  if (pred_off(pred)) return;
+  transpose_tile(in, out, M, N, blockIdx.y, blockIdx.x);
 }
 
 // Forward projection.  One workgroup owns up to V whole sinogram rows
@@ -297,22 +316,20 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
   }
 }
 
+// One forward group (gi indexes groups[]); acc is the dynamic LDS array.
 template <int V>
-__global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
-                                                  const float* __restrict__ imT,
-                                                  float* __restrict__ sino,
-                                                  const float4* __restrict__ geom,
-                                                  const double* __restrict__ rwt,
-                                                  const int4* __restrict__ groups,
-                                                  const int* __restrict__ order, int M, int N,
-                                                  int D, const float* __restrict__ yres,
-                                                  double* __restrict__ res_ws,
-                                                  const int32_t* pred) {
-  if (pred_off(pred)) return;
-  extern __shared__ float acc[];  // [V][4 waves][FWD_CH chains][L]
+__device__ __forceinline__ void fwd_group(const float* __restrict__ im,
+                                          const float* __restrict__ imT,
+                                          float* __restrict__ sino,
+                                          const float4* __restrict__ geom,
+                                          const double* __restrict__ rwt,
+                                          const int4* __restrict__ groups,
+                                          const int* __restrict__ order, int M, int N, int D,
+                                          const float* __restrict__ yres,
+                                          double* __restrict__ res_ws, int gi, float* acc) {
   __shared__ double red[4];
   const int L = D + 4;
-  const int4 grp = groups[blockIdx.x];
+  const int4 grp = groups[gi];
   const int cnt = grp.y;
   float Pxmin[V], P0[V], P1[V], wd[V];
   double rw[V];
@@ -353,10 +370,26 @@ __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
       }
     }
   }
-  if (res_ws) {  // per-workgroup partial of sum (y - A x)^2
+  if (res_ws) {  // per-group partial of sum (y - A x)^2
     res = block_sum_f64(res, red);
-    if (threadIdx.x == 0) res_ws[blockIdx.x] = res;
+    if (threadIdx.x == 0) res_ws[gi] = res;
   }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
+                                                  const float* __restrict__ imT,
+                                                  float* __restrict__ sino,
+                                                  const float4* __restrict__ geom,
+                                                  const double* __restrict__ rwt,
+                                                  const int4* __restrict__ groups,
+                                                  const int* __restrict__ order, int M, int N,
+                                                  int D, const float* __restrict__ yres,
+                                                  double* __restrict__ res_ws,
+                                                  const int32_t* pred) {
+  if (pred_off(pred)) return;
+  extern __shared__ float acc[];  // [V][4 waves][FWD_CH chains][L]
+  fwd_group<V>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws, blockIdx.x, acc);
 }
 
 // Adjoint (back projection).  One workgroup per ADJ_TI x ADJ_TJ pixel
@@ -381,20 +414,20 @@ struct __align__(16) ViewLds {
   int pad1;
 };
 
-__global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino,
-                                                  float* __restrict__ out,
-                                                  const float4* __restrict__ geom,
-                                                  const double* __restrict__ rwt,
-                                                  const float* __restrict__ clo, int M, int N,
-                                                  int A, int D, int W, AdjEpi epi,
-                                                  const int32_t* pred) {
-  if (pred_off(pred)) return;
+// One adjoint tile (tile row by, column bx; dot partial written at index
+// dot_i).
+__device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* __restrict__ out,
+                                         const float4* __restrict__ geom,
+                                         const double* __restrict__ rwt,
+                                         const float* __restrict__ clo, int M, int N, int A,
+                                         int D, int W, const AdjEpi& epi, int bx, int by,
+                                         int dot_i) {
   constexpr int VPW = ADJ_VC / 4;  // views staged per wave
   __shared__ float win[ADJ_VC * ADJ_WFAST];
   __shared__ ViewLds gv[ADJ_VC];
   __shared__ double red[4];
   const int tx = threadIdx.x % ADJ_TJ, ty = threadIdx.x / ADJ_TJ;
-  const int i0 = blockIdx.y * ADJ_TI, j0 = blockIdx.x * ADJ_TJ;
+  const int i0 = by * ADJ_TI, j0 = bx * ADJ_TJ;
   const int i = i0 + ty, j = j0 + tx;
   const bool inside = (i < M) && (j < N);
   const float fi = static_cast<float>(i), fj = static_cast<float>(j);
@@ -492,10 +525,22 @@ __global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino
     out[static_cast<size_t>(i) * N + j] = v;
     if (epi.dot_ws) dot = static_cast<double>(epi.p[static_cast<size_t>(i) * N + j]) * v;
   }
-  if (epi.dot_ws) {  // per-workgroup partial of sum p * out (CG curvature p^T A p)
+  if (epi.dot_ws) {  // per-tile partial of sum p * out (CG curvature p^T A p)
     dot = block_sum_f64(dot, red);
-    if (threadIdx.x == 0) epi.dot_ws[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+    if (threadIdx.x == 0) epi.dot_ws[dot_i] = dot;
   }
+}
+
+__global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino,
+                                                  float* __restrict__ out,
+                                                  const float4* __restrict__ geom,
+                                                  const double* __restrict__ rwt,
+                                                  const float* __restrict__ clo, int M, int N,
+                                                  int A, int D, int W, AdjEpi epi,
+                                                  const int32_t* pred) {
+  if (pred_off(pred)) return;
+  adj_tile(sino, out, geom, rwt, clo, M, N, A, D, W, epi, blockIdx.x, blockIdx.y,
+           blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // FBP row filter (_xray2d.py:187-194): the reference convolves each row
@@ -541,6 +586,114 @@ __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds
       weight_at(g.x, g.y, g.z, g.w, rwt[a], static_cast<float>(i), static_cast<float>(j));
   inds[e] = static_cast<int32_t>(wt.fl);
   wts[e] = wt.w;
+}
+
+// ------------------------------------------------------------- batched --
+// Ragged batches of slices in one launch: a persistent grid walks the work
+// items (slice, group / tile) and runs exactly the single-slice bodies
+// with that slice's tables, so every slice's result is bitwise the
+// single-slice result.  A slice whose predicate is 0 skips its items.
+
+__device__ __forceinline__ bool slice_off(const int32_t* pred, int pstride, int s) {
+  return pred != nullptr && pred[static_cast<int64_t>(s) * pstride] == 0;
+}
+
+__global__ __launch_bounds__(256) void transpose_batch_kernel(BatchTabs t,
+                                                              const float* __restrict__ im,
+                                                              float* __restrict__ imT,
+                                                              const int32_t* pred, int pstride) {
+  for (int it = blockIdx.x; it < t.nt; it += gridDim.x) {
+    const int4 w = t.titems[it];
+    if (slice_off(pred, pstride, w.x)) continue;
+    const int4 sh = t.shape[w.x];
+    const int64_t io = t.ioff[w.x];
+    transpose_tile(im + io, imT + io, sh.x, sh.y, w.y, w.z);
+    __syncthreads();
+  }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void fwd_batch_kernel(BatchTabs t, const float* __restrict__ im,
+                                                        const float* __restrict__ imT,
+                                                        float* __restrict__ sino,
+                                                        const float* __restrict__ yres,
+                                                        double* __restrict__ res_ws,
+                                                        const int32_t* pred, int pstride) {
+  extern __shared__ float acc[];
+  for (int it = blockIdx.x; it < t.nf; it += gridDim.x) {
+    const int2 w = t.fitems[it];
+    const int s = w.x;
+    if (slice_off(pred, pstride, s)) continue;
+    const int4 sh = t.shape[s];
+    const int64_t io = t.ioff[s], so = t.soff[s];
+    const int vo = t.voff[s], go = t.goff[s];
+    fwd_group<V>(im + io, imT + io, sino + so, t.geom + vo, t.rw + vo, t.groups + go,
+                 t.order + vo, sh.x, sh.y, sh.w, yres ? yres + so : nullptr,
+                 res_ws ? res_ws + go : nullptr, w.y, acc);
+    __syncthreads();  // the next item re-zeroes acc
+  }
+}
+
+__global__ __launch_bounds__(256) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
+                                                        float* __restrict__ out, BatchEpi be,
+                                                        const int32_t* pred, int pstride) {
+  for (int it = blockIdx.x; it < t.na; it += gridDim.x) {
+    const int2 w = t.aitems[it];
+    const int s = w.x;
+    if (slice_off(pred, pstride, s)) continue;
+    const int4 sh = t.shape[s], me = t.meta[s];
+    const int64_t io = t.ioff[s], so = t.soff[s];
+    const int vo = t.voff[s];
+    AdjEpi e;
+    e.kind = be.kind;
+    e.circular = be.circular;
+    e.rho = be.rho;
+    e.p = be.p ? be.p + io : nullptr;
+    e.z = be.z ? be.z + 2 * io : nullptr;
+    e.u = be.u ? be.u + 2 * io : nullptr;
+    e.dot_ws = be.dot_ws ? be.dot_ws + t.toff[s] : nullptr;
+    adj_tile(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y, sh.z, sh.w,
+             me.x, e, w.y % me.z, w.y / me.z, w.y);
+    __syncthreads();
+  }
+}
+
+int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im, float* imT,
+                  float* sino, const float* yres, double* res_ws, const int32_t* pred,
+                  int pstride, hipStream_t s) {
+  if (t.nt > 0) {
+    const unsigned g = grid > 0 ? std::min(grid, t.nt) : t.nt;
+    hipLaunchKernelGGL(transpose_batch_kernel, dim3(g), dim3(256), 0, s, t, im, imT, pred,
+                       pstride);
+    SCXR_LAUNCHED("xr2d batch transpose");
+  }
+  const unsigned g = grid > 0 ? std::min(grid, t.nf) : t.nf;
+  const size_t lds = fwd_lds_bytes(V, Dmax);
+  switch (V) {
+    case 1:
+      hipLaunchKernelGGL(fwd_batch_kernel<1>, dim3(g), dim3(256), lds, s, t, im, imT, sino, yres,
+                         res_ws, pred, pstride);
+      break;
+    case 2:
+      hipLaunchKernelGGL(fwd_batch_kernel<2>, dim3(g), dim3(256), lds, s, t, im, imT, sino, yres,
+                         res_ws, pred, pstride);
+      break;
+    default:
+      hipLaunchKernelGGL(fwd_batch_kernel<4>, dim3(g), dim3(256), lds, s, t, im, imT, sino, yres,
+                         res_ws, pred, pstride);
+      break;
+  }
+  SCXR_LAUNCHED("xr2d batch forward");
+  return SCXR_OK;
+}
+
+int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
+                  const BatchEpi& epi, const int32_t* pred, int pstride, hipStream_t s) {
+  const unsigned g = grid > 0 ? std::min(grid, t.na) : t.na;
+  hipLaunchKernelGGL(adj_batch_kernel, dim3(g), dim3(256), 0, s, t, sino, im, epi, pred,
+                     pstride);
+  SCXR_LAUNCHED("xr2d batch adjoint");
+  return SCXR_OK;
 }
 
 int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,

@@ -55,6 +55,65 @@ struct AdjEpi {
 
 void view_geometry(int A, const float* angles, const double* x0, const double* dx, double y0,
                    std::vector<float4>& geom, std::vector<double>& rw);
+
+// Host tables of one slice (geometry, forward groups, adjoint windows).
+struct HostTables {
+  std::vector<float4> geom;
+  std::vector<double> rw;
+  std::vector<int4> groups;
+  std::vector<int> order;
+  std::vector<float> clo;
+  int adj_w = 0;
+  bool any_colfast = false;
+};
+// vpg: views per forward group.
+int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
+                 double y0, const float* angles, int vpg);
+
+// Batched (ragged) slices: device tables of a batch plan.  Slice s has
+// shape (M, N, A, D) = shape[s]; its image starts at ioff[s] in packed
+// image vectors, its sinogram at soff[s], its views at voff[s] in the
+// packed view tables, its forward groups at goff[s] and its adjoint tiles
+// at toff[s].  Work items name (slice, group / tile).
+struct BatchTabs {
+  int ns = 0;
+  const int4* shape = nullptr;    // (M, N, A, D)
+  const int4* meta = nullptr;     // (adjoint window W, forward groups, tiles x, tiles y)
+  const int64_t* ioff = nullptr;
+  const int64_t* soff = nullptr;
+  const int* voff = nullptr;
+  const int* goff = nullptr;
+  const int* toff = nullptr;
+  const float4* geom = nullptr;
+  const double* rw = nullptr;
+  const int4* groups = nullptr;
+  const int* order = nullptr;
+  const float* clo = nullptr;
+  const int2* fitems = nullptr;   // forward items (slice, group)
+  const int2* aitems = nullptr;   // adjoint items (slice, tile)
+  const int4* titems = nullptr;   // transpose items (slice, tile row, tile col, 0)
+  int nf = 0, na = 0, nt = 0;
+};
+
+// Epilogue arguments of the batched adjoint (packed vectors; z, u are
+// (2, M, N) per slice at offset 2 ioff[s]; dot partials per tile at toff).
+struct BatchEpi {
+  int kind = EPI_NONE;
+  int circular = 0;
+  float rho = 0.0f;
+  const float* p = nullptr;
+  const float* z = nullptr;
+  const float* u = nullptr;
+  double* dot_ws = nullptr;
+};
+
+// Launchers (grid = persistent workgroup count, 0 = one per item).  pred
+// (nullable) holds one predicate per slice at pred[s * pstride].
+int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im, float* imT,
+                  float* sino, const float* yres, double* res_ws, const int32_t* pred,
+                  int pstride, hipStream_t s);
+int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
+                  const BatchEpi& epi, const int32_t* pred, int pstride, hipStream_t s);
 int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx, double y0,
            const float* angles);
 void destroy(Plan& p);
