@@ -13,7 +13,17 @@ configured-tolerance run (CG tol 1e-4, 100 iterations, where warm-started
 CG stops early) is reported beside it with its mean CG iteration count.
 
 Prints one JSON line with the roofline of the dominant projector kernel
-(HIP events) and the CPU baseline of the C/OpenMP restatement.
+(HIP events) and the CPU baseline of the C/OpenMP restatement.  "extras"
+reports the other workloads of the same hot path on the same GPU (skip
+with --no-extras): the 1024^2 CircularConvolve + AcceleratedPGM-TV
+deconvolution (200 iterations), 3D ADMM-TV on a 256^3 volume (360 views,
+detector 368 x 256), 256 ragged slices batched in one launch per
+projection (autotuned, graph-captured ADMM-TV), and the forward + backward
+pass of the 20-step unrolled APGM-TV reconstruction (512^2, 360 views).
+
+``--kernels`` runs only the projector launches (2D headline shape, 3D
+slice shape) plus a 64 MiB copy for counter calibration: the workload of
+the rocprofv3 counter runs.
 """
 
 from __future__ import annotations
@@ -31,6 +41,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-instructions / s (256 CUs x 64 lanes x 2.4 GHz)
+
+# HBM traffic per launch of the 2D projector at 512^2 / 720 views / 724 bins
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled
+# by the counters of a 64 MiB copy (FETCH_SIZE reads 0.5x the bytes moved,
+# WRITE_SIZE 1.0x): forward = image transpose + projection.
+TRAFFIC_BYTES = {
+    "forward": 1024 * ((524.2 + 8366.2) * 2.0 + (1024.0 + 2047.5) * 1.0),
+    "adjoint": 1024 * (5118.5 * 2.0 + 1024.0 * 1.0),
+}
+# VALU instructions (lane-instructions) per (pixel, view) interaction,
+# rocprofv3 --pmc SQ_INSTS_VALU x 64 / interactions, same workload.
+VALU_PER_INTERACTION = {"forward": 22.63, "adjoint": 25.09}
 
 
 def log(*a):
@@ -145,6 +167,213 @@ def cpu_baseline(y, N, A, D, lam, rho, cg_maxiter, iters):
                       f"{dt:.1f} s"}
 
 
+def extra_deconv(iters=200):
+    """1024^2 ellipse phantom, 15 x 15 Gaussian PSF (sigma 2, centred at
+    (7, 7)), 1 % noise; AcceleratedPGM with 2e-2 * IsotropicTVNorm, L0 =
+    max |H|^2."""
+    import torch
+
+    from scico_amd import functional, loss, phantom
+    from scico_amd.linop import CircularConvolve
+    from scico_amd.optimize import AcceleratedPGM
+
+    N = 1024
+    r = np.arange(15) - 7
+    psf = np.exp(-(r[:, None] ** 2 + r[None, :] ** 2) / (2 * 2.0**2))
+    psf = (psf / psf.sum()).astype(np.float32)
+    A = CircularConvolve(psf, (N, N), h_center=[7, 7])
+    xgt = torch.as_tensor(phantom.ellipse_phantom(N, 20, seed=0), device="cuda")
+    y = torch.as_tensor(phantom.noisy(A(xgt).cpu().numpy(), 0.01, seed=1), device="cuda")
+    L0 = float(torch.max(torch.abs(A.h_dft.to(torch.complex128)) ** 2))
+    f = loss.SquaredL2Loss(y=y, A=A)
+    g = 2e-2 * functional.IsotropicTVNorm()
+
+    def make(n):
+        return AcceleratedPGM(f=f, g=g, L0=L0, x0=torch.zeros((N, N), device="cuda"), maxiter=n)
+
+    make(3).solve()  # warm up
+    torch.cuda.synchronize()
+    s = make(iters)
+    t0 = time.perf_counter()
+    x = s.solve()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = torch.empty_like(xgt)
+    for _ in range(3):
+        A._eval(xgt, out=out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        A._eval(xgt, out=out)
+    e1.record()
+    e1.synchronize()
+    snr = 10 * np.log10(float(torch.var(xgt.double())) /
+                        float(torch.mean((x.double() - xgt.double()) ** 2)))
+    return {"workload": "apgm_tv_deconv_1024x1024_psf15_sigma2", "iters": iters,
+            "seconds": round(dt, 4), "iters_per_s": round(iters / dt, 2),
+            "cconv_apply_us": round(e0.elapsed_time(e1) / 20 * 1000, 2),
+            "snr_db": round(snr, 3), "L0": L0, "lambda": 2e-2}
+
+
+def extra_3d(steps=5, warmup=2):
+    """256^3 ellipsoid phantom (30), 360 views about axis 1, detector
+    368 x 256, 1 % noise; ADMM with 2e-2 * L1Norm (anisotropic TV) of
+    FiniteDifference(append=0), rho 1, CG 10 steps (fixed work)."""
+    import torch
+
+    from scico_amd import functional, linop, loss, phantom
+    from scico_amd.linop.xray import XRayTransform3D
+    from scico_amd.optimize import ADMM, LinearSubproblemSolver
+
+    N, V, det = 256, 360, (368, 256)
+    M = XRayTransform3D.matrices_from_euler_angles((N, N, N), det, "Y", phantom.angles(V))
+    H = XRayTransform3D((N, N, N), M, det)
+    vol = torch.as_tensor(phantom.ellipsoid_phantom((N, N, N), 30, seed=0), device="cuda")
+    y = H(vol)
+    y = y + 0.01 * float(y.max()) * torch.randn(y.shape, device="cuda",
+                                                generator=torch.Generator("cuda").manual_seed(1))
+    s = ADMM(f=loss.SquaredL2Loss(y=y, A=H), g_list=[2e-2 * functional.L1Norm()],
+             C_list=[linop.FiniteDifference((N, N, N), append=0)], rho_list=[1.0],
+             maxiter=steps + warmup, subproblem_solver=LinearSubproblemSolver(
+                 cg_kwargs={"tol": 0.0, "maxiter": 10}))
+    assert s._fused is not None
+    dt, hist = run_timed(s, steps, warmup, True)
+    kt = time_kernels(H, reps=5)
+    inter = float(N) ** 3 * V
+    return {"workload": "admm_tv3d_256cube_360views_det368x256_cg10_fixedwork",
+            "steps": steps, "ms_per_iter": round(1000 * dt / steps, 3),
+            "iters_per_s": round(steps / dt, 3),
+            "projected_50_iters_s": round(50 * dt / steps, 3),
+            "forward_ms": round(kt["forward"] * 1e3, 3), "adjoint_ms": round(kt["adjoint"] * 1e3, 3),
+            "interactions_per_s": {k: round(inter / v, 1) for k, v in kt.items()},
+            "last_objective": float(hist[-1, 0])}
+
+
+def extra_batch(steps=3, warmup=1):
+    """256 slices: N from {128, 192, 256, 384, 512}, 90..720 views,
+    det_count = ceil(sqrt(2) N), ellipse phantoms (10) + 1 % noise;
+    autotuned persistent projector; ADMM-TV (lam 2e-2, rho 1, CG 10 fixed
+    work) on all slices per graph replay."""
+    import torch
+
+    from scico_amd import phantom
+    from scico_amd.batch import ADMMTVBatch, XRayTransform2DBatch
+
+    rng = np.random.default_rng(8)
+    shapes, angles = [], []
+    for _ in range(256):
+        n = int(rng.choice([128, 192, 256, 384, 512]))
+        shapes.append((n, n))
+        angles.append(phantom.angles(int(rng.integers(90, 721))))
+    B = XRayTransform2DBatch(shapes, angles)
+    t0 = time.perf_counter()
+    tuned = B.autotune(vpgs=(1, 2), grids=(0, 2, 4), orders=(0, 1, 2), reps=2)
+    t_tune = time.perf_counter() - t0
+    xs = [phantom.ellipse_phantom(sh[0], 10, seed=k) for k, sh in enumerate(shapes)]
+    y = B.project(B.pack_images(xs))
+    y = y + 0.01 * float(y.max()) * torch.randn(y.shape, device="cuda",
+                                                generator=torch.Generator("cuda").manual_seed(1))
+    bs = ADMMTVBatch(B, y, 2e-2, 1.0, cg_maxiter=10, cg_tol=0.0, hist_cap=steps + warmup + 4)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        bs.enqueue()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        bs.enqueue()
+    for _ in range(warmup):
+        graph.replay()
+    bs.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    inter = float(sum(s[0] * s[1] * len(a) for s, a in zip(shapes, angles)))
+    return {"workload": "batched_256_ragged_slices_admm_tv_cg10_fixedwork", "slices": 256,
+            "interactions_per_pass": inter, "autotuned": {k: tuned[k] for k in
+                                                          ("vpg", "grid", "order")},
+            "autotune_seconds": round(t_tune, 2),
+            "projector_pass_ms": round(tuned["ms"], 3),
+            "projector_interactions_per_s": round(2 * inter / (tuned["ms"] * 1e-3), 1),
+            "steps": steps, "ms_per_iter": round(1000 * dt / steps, 2),
+            "slice_iters_per_s": round(256 * steps / dt, 1),
+            "projected_50_iters_s": round(50 * dt / steps, 2)}
+
+
+def extra_unrolled(K=20):
+    """512^2, 360 views (det_count 725), 20-step unrolled APGM-TV with a
+    per-pixel TV weight map, checkpoints every 5 steps: forward and
+    forward + backward (gradients w.r.t. the sinogram and the weights)."""
+    import torch
+
+    from scico_amd import autograd as AG
+    from scico_amd import phantom
+    from scico_amd.linop import operator_norm
+    from scico_amd.linop.xray import XRayTransform2D
+
+    N, V = 512, 360
+    H = XRayTransform2D((N, N), phantom.angles(V))
+    xt = torch.as_tensor(phantom.ellipse_phantom(N, 20, seed=0), device="cuda")
+    y0 = H(xt)
+    L = 1.01 * float(operator_norm(H, maxiter=30)) ** 2
+
+    def once(grad):
+        y = y0.clone().requires_grad_(grad)
+        w = torch.ones((N, N), device="cuda", requires_grad=grad)
+        x = AG.unrolled_apgm_tv(H, y, w, L, 2e-2, K=K, checkpoint_every=5)
+        if grad:
+            ((x - xt) ** 2).sum().backward()
+
+    for g in (False, True):
+        once(g)
+    out = {}
+    for g in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            once(g)
+        torch.cuda.synchronize()
+        out["fwd_bwd_ms" if g else "fwd_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
+    return {"workload": "unrolled_apgm_tv_512x512_360views_K20_ckpt5", **out}
+
+
+def kernels_only():
+    """Projector launches only (counter runs): 10 x 2D forward + adjoint at
+    512^2 / 720 / 724, 10 x 64 MiB copy (calibration), 3 x 3D forward +
+    adjoint at 256^3 / 360 / (368, 256)."""
+    import torch
+
+    from scico_amd import _vec, phantom
+    from scico_amd.linop.xray import XRayTransform2D, XRayTransform3D
+
+    H = XRayTransform2D((512, 512), phantom.angles(720), det_count=724)
+    x = torch.rand((512, 512), device="cuda")
+    y = torch.rand((720, 724), device="cuda")
+    s, b = torch.empty_like(y), torch.empty_like(x)
+    big = torch.rand(16 << 20, device="cuda")
+    big2 = torch.empty_like(big)
+    for _ in range(10):
+        H.project(x, out=s)
+    for _ in range(10):
+        H.back_project(y, out=b)
+    for _ in range(10):
+        _vec.copy(big, out=big2)
+    M = XRayTransform3D.matrices_from_euler_angles((256, 256, 256), (368, 256), "Y",
+                                                    phantom.angles(360))
+    H3 = XRayTransform3D((256, 256, 256), M, (368, 256))
+    v = torch.rand((256, 256, 256), device="cuda")
+    p = torch.rand((360, 368, 256), device="cuda")
+    for _ in range(3):
+        H3.project(v)
+        H3.back_project(p)
+    torch.cuda.synchronize()
+    log("bench: kernel launches done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +382,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--kernels", action="store_true",
+                    help="projector launches only (for counter runs)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -161,6 +393,10 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.kernels:
+        kernels_only()
+        return
 
     N, A, D = 512, 720, 724
     lam, rho, cg_maxiter = 2e-2, 1.0, 10
@@ -193,18 +429,39 @@ def main():
     dom = max(kt, key=kt.get)
     t_dom = kt[dom]
     achieved = bytes_per / t_dom / 1e9
+    vpi = VALU_PER_INTERACTION[dom]
     roofline = {
-        "kernel": f"xr2d {dom}",
+        "kernel": "xr2d " + ("forward (image transpose + projection)" if dom == "forward"
+                             else "adjoint"),
         "bound": "hbm",
         "achieved": round(achieved, 3),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6),
-        "traffic": None,
+        "traffic": round(TRAFFIC_BYTES[dom]),
+        "algorithmic_bytes": bytes_per,
         "avg_launch_us": {k: round(v * 1e6, 2) for k, v in kt.items()},
         "interactions_per_s": {k: round(inter / v, 1) for k, v in kt.items()},
+        # the projector is bound by vector-ALU issue, not HBM: its
+        # lane-instruction rate against the VALU peak
+        "valu": None if vpi is None else {
+            "instr_per_interaction": vpi,
+            "achieved": round(inter * vpi / t_dom, 1),
+            "peak": VALU_PEAK,
+            "unit": "lane-instr/s",
+            "frac": round(inter * vpi / t_dom / VALU_PEAK, 4)},
     }
     log(f"bench: forward {kt['forward']*1e6:.1f} us, adjoint {kt['adjoint']*1e6:.1f} us per launch")
+
+    extras = None
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("deconvolution", extra_deconv), ("xray3d", extra_3d),
+                         ("batched", extra_batch), ("unrolled_grad", extra_unrolled)):
+            log(f"bench: extra {name}")
+            t0 = time.perf_counter()
+            extras[name] = fn()
+            log(f"bench: extra {name} done in {time.perf_counter() - t0:.1f} s: {extras[name]}")
 
     cpu = None
     if not args.no_cpu and args.cpu_iters > 0:
@@ -233,6 +490,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(ips / cpu["value"], 1) if cpu else None,
+        "extras": extras,
     }
     print(json.dumps(out))
 

@@ -34,18 +34,32 @@ def ellipse_phantom(N: int, n_ellipses: int = 10, seed: int = 0, M: int | None =
 
 
 def ellipsoid_phantom(shape, n_ellipsoids: int = 30, seed: int = 0) -> np.ndarray:
+    """3D analogue of :func:`ellipse_phantom`: centres U[-0.7, 0.7]^3,
+    semi-axes U[0.05, 0.4]^3, rotation Rz(a) Ry(b) Rz(g) with each angle
+    U[0, pi), intensities U[0.1, 1]; voxel centres on [-1, 1]^3.  Each
+    ellipsoid is rasterised inside its bounding box only."""
     rng = np.random.default_rng(seed)
     grids = [-1.0 + (2.0 * np.arange(s) + 1.0) / s for s in shape]
     vol = np.zeros(shape, dtype=np.float64)
-    Z = np.stack(np.meshgrid(*grids, indexing="ij"), axis=-1)
     for _ in range(n_ellipsoids):
         c = rng.uniform(-0.7, 0.7, size=3)
         ax = rng.uniform(0.05, 0.4, size=3)
         a, b, g = rng.uniform(0.0, np.pi, size=3)
         val = rng.uniform(0.1, 1.0)
         R = _rz(a) @ _ry(b) @ _rz(g)
+        r = float(np.max(ax))
+        sl = []
+        for k, gk in enumerate(grids):
+            lo = int(np.searchsorted(gk, c[k] - r, side="left"))
+            hi = int(np.searchsorted(gk, c[k] + r, side="right"))
+            sl.append(slice(lo, hi))
+        if any(t.stop <= t.start for t in sl):
+            continue
+        Z = np.stack(np.meshgrid(grids[0][sl[0]], grids[1][sl[1]], grids[2][sl[2]],
+                                 indexing="ij"), axis=-1)
         d = (Z - c) @ R
-        vol[np.sum((d / ax) ** 2, axis=-1) <= 1.0] += val
+        inside = np.sum((d / ax) ** 2, axis=-1) <= 1.0
+        vol[tuple(sl)][inside] += val
     return vol.astype(np.float32)
 
 
