@@ -47,8 +47,8 @@ VALU_PEAK = 256 * 64 * 2.4e9  # lane-instructions / s (256 CUs x 64 lanes x 2.4 
 # by the counters of a 64 MiB copy (FETCH_SIZE reads 0.5x the bytes moved,
 # WRITE_SIZE 1.0x): forward = image transpose + projection.
 TRAFFIC_BYTES = {
-    "forward": 1024 * ((524.2 + 8366.2) * 2.0 + (1024.0 + 2047.5) * 1.0),
-    "adjoint": 1024 * (5118.5 * 2.0 + 1024.0 * 1.0),
+    "forward": 1024 * ((524.3 + 8468.5) * 2.0 + (1024.0 + 2047.5) * 1.0),
+    "adjoint": 1024 * (5112.0 * 2.0 + 1024.0 * 1.0),
 }
 # VALU instructions (lane-instructions) per (pixel, view) interaction,
 # rocprofv3 --pmc SQ_INSTS_VALU x 64 / interactions, same workload.

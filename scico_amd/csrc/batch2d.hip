@@ -106,22 +106,49 @@ int create(Batch& b, int ns, const int32_t* dims, const float* angles, const dou
   std::vector<int> G, woff, blk_seg, blk_first;
   int64_t io = 0, so = 0;
   int vo = 0, go = 0, to = 0, wo = 0;
-  const float* ang = angles;
+  // Tables of every slice; one adjoint tile shape for the whole batch (two
+  // pixels per thread unless a slice's window needs one).
+  std::vector<xr2d::HostTables> tabs(ns);
+  std::vector<const float*> angp(ns);
+  {
+    const float* ang = angles;
+    int ppt = 2;
+    for (int s = 0; s < ns; ++s) {
+      const int M = dims[4 * s], N = dims[4 * s + 1], A = dims[4 * s + 2], D = dims[4 * s + 3];
+      SCXR_REQUIRE(M > 0 && N > 0 && A > 0 && D > 0, SCXR_ERR_SHAPE,
+                   "slice %d: non-positive size (M=%d N=%d A=%d D=%d)", s, M, N, A, D);
+      const double* g5 = geo + 5 * s;
+      angp[s] = ang;
+      const int rc = xr2d::build_tables(tabs[s], M, N, A, D, g5, g5 + 2, g5[4], ang, vpg, 0);
+      if (rc) {
+        const std::string msg = scxr_last_error();
+        return fail(rc, "slice %d: %s", s, msg.c_str());
+      }
+      ppt = std::min(ppt, tabs[s].adj_ppt);
+      ang += A;
+    }
+    for (int s = 0; s < ns; ++s) {
+      if (tabs[s].adj_ppt == ppt) continue;
+      const double* g5 = geo + 5 * s;
+      const int rc = xr2d::build_tables(tabs[s], dims[4 * s], dims[4 * s + 1], dims[4 * s + 2],
+                                        dims[4 * s + 3], g5, g5 + 2, g5[4], angp[s], vpg, ppt);
+      if (rc) return rc;
+    }
+    b.bt.ppt = ppt;
+  }
+  {
+    int Dm = 0;
+    for (int s = 0; s < ns; ++s) Dm = std::max(Dm, dims[4 * s + 3]);
+    b.bt.ch = xr2d::fwd_chains(vpg, Dm);
+    SCXR_REQUIRE(b.bt.ch > 0, SCXR_ERR_UNSUPPORTED,
+                 "det_count %d too large for the forward projector with vpg %d", Dm, vpg);
+  }
   for (int s = 0; s < ns; ++s) {
     const int M = dims[4 * s], N = dims[4 * s + 1], A = dims[4 * s + 2], D = dims[4 * s + 3];
-    SCXR_REQUIRE(M > 0 && N > 0 && A > 0 && D > 0, SCXR_ERR_SHAPE,
-                 "slice %d: non-positive size (M=%d N=%d A=%d D=%d)", s, M, N, A, D);
-    const double* g5 = geo + 5 * s;
-    xr2d::HostTables h;
-    const int rc = xr2d::build_tables(h, M, N, A, D, g5, g5 + 2, g5[4], ang, vpg);
-    if (rc) {
-      const std::string msg = scxr_last_error();
-      return fail(rc, "slice %d: %s", s, msg.c_str());
-    }
-    ang += A;
+    const xr2d::HostTables& h = tabs[s];
     b.any_colfast = b.any_colfast || h.any_colfast;
     b.Dmax = std::max(b.Dmax, D);
-    const int tx = static_cast<int>(ceil_div(N, xr2d::ADJ_TJ));
+    const int tx = static_cast<int>(ceil_div(N, xr2d::ADJ_TJ * b.bt.ppt));
     const int ty = static_cast<int>(ceil_div(M, xr2d::ADJ_TI));
     b.shape.push_back(make_int4(M, N, A, D));
     b.meta.push_back(make_int4(h.adj_w, static_cast<int>(h.groups.size()), tx, ty));

@@ -92,4 +92,18 @@ __device__ __forceinline__ double all_sum_f64(const double* __restrict__ parts, 
   return r;
 }
 
+// Raw buffer access (gfx9-family buffer word 3): a wave-uniform base and
+// row offset live in SGPRs, the lane offset in one VGPR, so a load costs no
+// per-lane address arithmetic; loads at or beyond `bytes` return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0), bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ float bload1(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 }  // namespace scxr

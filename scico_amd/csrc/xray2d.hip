@@ -58,7 +58,7 @@ void view_geometry(int A, const float* angles, const double* x0, const double* d
 }
 
 int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
-                 double y0, const float* angles, int vpg) {
+                 double y0, const float* angles, int vpg, int ppt) {
   view_geometry(A, angles, x0, dx, y0, h.geom, h.rw);
   const std::vector<float4>& geom = h.geom;
   for (int a = 0; a < A; ++a)
@@ -75,7 +75,7 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
   for (int a = 0; a < A; ++a)
     (std::fabs(geom[a].z) >= std::fabs(geom[a].y) ? rows : cols).push_back(a);
   h.any_colfast = !cols.empty();
-  SCXR_REQUIRE(fwd_lds_bytes(vpg, D) <= FWD_LDS_MAX, SCXR_ERR_UNSUPPORTED,
+  SCXR_REQUIRE(fwd_chains(vpg, D) > 0, SCXR_ERR_UNSUPPORTED,
                "det_count %d too large for the LDS-resident forward projector", D);
   h.order.clear();
   h.groups.clear();
@@ -101,18 +101,32 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
 
   // Adjoint: window origin offset per view (see adj_tile): a lower bound
   // of floor(Px) - 1 over any tile; the window spans the projected tile
-  // extent plus the lower margin, the rounding of Px and bin ind + 1.
-  h.clo.assign(A, 0.0f);
-  int wmax = 0;
-  for (int a = 0; a < A; ++a) {
-    const double P0 = geom[a].y, P1 = geom[a].z;
-    h.clo[a] = static_cast<float>(static_cast<double>(geom[a].x) +
+  // extent plus the lower margin, the rounding of Px and bin ind + 1.  Two
+  // pixels per thread (16 x 32 tiles) when that window fits, else one.
+  auto windows = [&](int pp, std::vector<float>& clo, int& wmax) {
+    const int TJ = ADJ_TJ * pp;
+    clo.assign(A, 0.0f);
+    wmax = 0;
+    for (int a = 0; a < A; ++a) {
+      const double P0 = geom[a].y, P1 = geom[a].z;
+      clo[a] = static_cast<float>(static_cast<double>(geom[a].x) +
                                   std::min(P0, 0.0) * (ADJ_TI - 1) +
-                                  std::min(P1, 0.0) * (ADJ_TJ - 1) - 1.0);
-    const double ext = (ADJ_TI - 1) * std::fabs(P0) + (ADJ_TJ - 1) * std::fabs(P1);
-    wmax = std::max(wmax, static_cast<int>(std::ceil(ext)) + 5);
+                                  std::min(P1, 0.0) * (TJ - 1) - 1.0);
+      const double ext = (ADJ_TI - 1) * std::fabs(P0) + (TJ - 1) * std::fabs(P1);
+      wmax = std::max(wmax, static_cast<int>(std::ceil(ext)) + 5);
+    }
+  };
+  if (ppt == 0) {
+    windows(2, h.clo, h.adj_w);
+    h.adj_ppt = 2;
+    if (h.adj_w > ADJ_WFAST) {
+      windows(1, h.clo, h.adj_w);
+      h.adj_ppt = 1;
+    }
+  } else {
+    windows(ppt, h.clo, h.adj_w);
+    h.adj_ppt = ppt;
   }
-  h.adj_w = wmax;
   SCXR_REQUIRE(h.adj_w <= ADJ_WFAST, SCXR_ERR_GEOM,
                "projected pixel footprint too large for the adjoint tile (%d bins > %d); "
                "pixel size dx is too large relative to the detector spacing",
@@ -128,10 +142,12 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   p.D = D;
   // Views per workgroup: enough workgroups to fill 256 CUs at least twice.
   p.vpg = A >= 2048 ? 4 : (A >= 1024 ? 2 : 1);
-  while (p.vpg > 1 && fwd_lds_bytes(p.vpg, D) > FWD_LDS_MAX) p.vpg /= 2;
+  while (p.vpg > 1 && fwd_chains(p.vpg, D) == 0) p.vpg /= 2;
+  p.fwd_ch = fwd_chains(p.vpg, D);
   HostTables h;
-  const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg);
+  const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg, 0);
   if (rc) return rc;
+  p.adj_ppt = h.adj_ppt;
   p.host_geom = h.geom;
   p.any_colfast = h.any_colfast;
   p.n_groups = static_cast<int>(h.groups.size());
@@ -238,86 +254,138 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 // plain LDS read-modify-write is race free.  The S phases of a wave run in
 // program order; LDS executes a wave's accesses in order, and a compiler
 // fence keeps the phases' memory operations in that order.
-// Each wave runs two independent chains (lines s and s + 4, each with its
-// own private row copy) so that one chain's LDS round trip overlaps the
+// Each wave runs CH independent chains (lines s, s + 4, ..., each with its
+// own private row copy) so that the chains' LDS round trips overlap each
 // other's arithmetic.  Px = (Pxmin + P0 * i) + P1 * j is evaluated in the
 // reference order with the line-constant product hoisted: for row-fast
 // views t = Pxmin + P0 * i is constant along the line, for column-fast
 // views t = P1 * j is.
-template <int V, bool COLFAST>
+// Accumulate one phase (lanes at f = c0 + S lane + r) of the CH lines.
+template <int V, int CH, bool COLFAST>
+__device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
+                                          const float (&t)[CH][V], const float* val,
+                                          float ff, int cnt, const float* Pxmin,
+                                          const float* P0, const float* P1, const double* rw,
+                                          float Dmax) {
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    if (V == 1 || v < cnt) {
+      // the lane-axis term is shared by the chains
+      const float tl = COLFAST ? __fadd_rn(Pxmin[v], __fmul_rn(P0[v], ff)) : __fmul_rn(P1[v], ff);
+      int k[CH];
+      float w[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const float P = COLFAST ? __fadd_rn(tl, t[c][v]) : __fadd_rn(t[c][v], tl);
+        const float fl = floorf(P);
+        w[c] = fminf(div_rn(__fsub_rn(1.0f, __fsub_rn(P, fl)), rw[v]), 1.0f);
+        k[c] = row[c][v] + static_cast<int>(__builtin_amdgcn_fmed3f(fl, -2.0f, Dmax));
+      }
+      float a0[CH], a1[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        a0[c] = acc[k[c]];
+        a1[c] = acc[k[c] + 1];
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        acc[k[c]] = fmaf(val[c], w[c], a0[c]);
+        acc[k[c] + 1] = fmaf(val[c], __fsub_rn(1.0f, w[c]), a1[c]);
+      }
+    }
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Forward projection of a view group by one wave.  Chain c of the wave
+// streams lines s0 + 4c (s0 = wave + 4 CH g) along the fast axis; lane l
+// takes pixels f = c0 + S l + r in phase r, so the lanes of one phase hit
+// bins at least two apart and each chain's LDS row is updated without
+// atomics.  The walk over (line group, block c0, phase r) is flat and the
+// image values of the next phase are loaded before the current phase is
+// accumulated.
+template <int V, int CH, bool COLFAST>
 __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* acc, int L,
                                          int cnt, int S, const float* Pxmin, const float* P0,
                                          const float* P1, const double* rw, int M, int N,
                                          int D) {
-  static_assert(FWD_CH == 2, "two chains per wave");
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const unsigned Lf = COLFAST ? M : N;  // fast (lane) axis length
   const int Ls = COLFAST ? N : M;
+  if (wave >= Ls) return;
   const float Dmax = static_cast<float>(D);
   const unsigned span = 64u * S;
   const unsigned loff = static_cast<unsigned>(S * lane);
-  int rowa[V], rowb[V];
+  int row[CH][V];
 #pragma unroll
-  for (int v = 0; v < V; ++v) {
-    rowa[v] = (v * 4 + wave) * 2 * L + 2;
-    rowb[v] = rowa[v] + L;
-  }
-  for (int sa = wave; sa < Ls; sa += 8) {
-    const int sb = sa + 4;
-    const bool has_b = sb < Ls;
-    const float* __restrict__ ra = src + static_cast<size_t>(sa) * Lf;
-    const float* __restrict__ rb = src + static_cast<size_t>(has_b ? sb : sa) * Lf;
-    const float scale_b = has_b ? 1.0f : 0.0f;
-    float ta[V], tb[V];
+  for (int v = 0; v < V; ++v)
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const float fa = static_cast<float>(sa), fb = static_cast<float>(sb);
-      ta[v] = COLFAST ? __fmul_rn(P1[v], fa) : __fadd_rn(Pxmin[v], __fmul_rn(P0[v], fa));
-      tb[v] = COLFAST ? __fmul_rn(P1[v], fb) : __fadd_rn(Pxmin[v], __fmul_rn(P0[v], fb));
+    for (int c = 0; c < CH; ++c) row[c][v] = ((v * 4 + wave) * CH + c) * L + 2;
+  // Line group s0: chain c streams line s0 + 4c through its own buffer;
+  // lanes past the line end (and lines past the image) read zeros, which
+  // add nothing.
+  __amdgpu_buffer_rsrc_t rs[CH];
+  auto set_lines = [&](int s0) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int sc = s0 + 4 * c;
+      const bool has = sc < Ls;
+      rs[c] = make_rsrc(src + static_cast<size_t>(has ? sc : s0) * Lf,
+                        has ? static_cast<int>(Lf * 4u) : 0);
     }
-    for (unsigned c0 = 0; c0 < Lf; c0 += span) {
-      unsigned f = c0 + loff;
-      float ff = static_cast<float>(f);
-      for (int r = 0; r < S; ++r, ++f, ff += 1.0f) {
-        if (f < Lf) {
-          const float va = ra[f];
-          const float vb = __fmul_rn(rb[f], scale_b);
+  };
+  // The walk over (line group, block, phase) is flat; the image values of
+  // the next phase are loaded before the current phase is accumulated.
+  int s0 = wave, sn = wave;
+  unsigned c0 = 0, cn = 0;
+  int r = 0, rn = 0;
+  set_lines(sn);
+  float nxt[CH];
 #pragma unroll
-          for (int v = 0; v < V; ++v) {
-            if (V == 1 || v < cnt) {
-              float Pa, Pb;
-              if (COLFAST) {
-                const float t = __fadd_rn(Pxmin[v], __fmul_rn(P0[v], ff));
-                Pa = __fadd_rn(t, ta[v]);
-                Pb = __fadd_rn(t, tb[v]);
-              } else {
-                const float t = __fmul_rn(P1[v], ff);
-                Pa = __fadd_rn(ta[v], t);
-                Pb = __fadd_rn(tb[v], t);
-              }
-              const float fla = floorf(Pa), flb = floorf(Pb);
-              const float wa = fminf(div_rn(__fsub_rn(1.0f, __fsub_rn(Pa, fla)), rw[v]), 1.0f);
-              const float wb = fminf(div_rn(__fsub_rn(1.0f, __fsub_rn(Pb, flb)), rw[v]), 1.0f);
-              const int ka = rowa[v] + static_cast<int>(__builtin_amdgcn_fmed3f(fla, -2.0f, Dmax));
-              const int kb = rowb[v] + static_cast<int>(__builtin_amdgcn_fmed3f(flb, -2.0f, Dmax));
-              const float a0 = acc[ka], a1 = acc[ka + 1];
-              const float b0 = acc[kb], b1 = acc[kb + 1];
-              acc[ka] = __fadd_rn(a0, __fmul_rn(va, wa));
-              acc[ka + 1] = __fadd_rn(a1, __fmul_rn(va, __fsub_rn(1.0f, wa)));
-              acc[kb] = __fadd_rn(b0, __fmul_rn(vb, wb));
-              acc[kb + 1] = __fadd_rn(b1, __fmul_rn(vb, __fsub_rn(1.0f, wb)));
-            }
-          }
-        }
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_wave_barrier();
+  for (int c = 0; c < CH; ++c) nxt[c] = bload1(rs[c], (loff) * 4u, 0);
+  float t[CH][V];
+  int tline = -1;
+  while (true) {
+    float val[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) val[c] = nxt[c];
+    s0 = sn;
+    c0 = cn;
+    r = rn;
+    // advance to the next phase
+    if (++rn == S) {
+      rn = 0;
+      cn += span;
+      if (cn >= Lf) {
+        cn = 0;
+        sn += 4 * CH;
+        if (sn < Ls) set_lines(sn);
       }
     }
+    if (sn < Ls) {
+      const unsigned fo = (cn + loff + rn) * 4u;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) nxt[c] = bload1(rs[c], fo, 0);
+    }
+    if (s0 != tline) {
+      tline = s0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const float fs = static_cast<float>(s0 + 4 * c);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          t[c][v] = COLFAST ? __fmul_rn(P1[v], fs) : __fadd_rn(Pxmin[v], __fmul_rn(P0[v], fs));
+      }
+    }
+    const float ff = static_cast<float>(c0 + loff + r);
+    fwd_phase<V, CH, COLFAST>(acc, row, t, val, ff, cnt, Pxmin, P0, P1, rw, Dmax);
+    if (sn >= Ls) break;
   }
 }
 
 // One forward group (gi indexes groups[]); acc is the dynamic LDS array.
-template <int V>
+template <int V, int CH>
 __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
                                           const float* __restrict__ imT,
                                           float* __restrict__ sino,
@@ -345,23 +413,23 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
     wd[v] = g.w;
     rw[v] = rwt[a];
   }
-  for (int e = threadIdx.x; e < V * 4 * FWD_CH * L; e += 256) acc[e] = 0.0f;
+  for (int e = threadIdx.x; e < V * 4 * CH * L; e += 256) acc[e] = 0.0f;
   __syncthreads();
   if (grp.z & 1)
-    fwd_body<V, true>(imT, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+    fwd_body<V, CH, true>(imT, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
   else
-    fwd_body<V, false>(im, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+    fwd_body<V, CH, false>(im, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
   __syncthreads();
   double res = 0.0;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     if (v < cnt) {
-      const float* r = acc + v * 4 * FWD_CH * L + 2;
+      const float* r = acc + v * 4 * CH * L + 2;
       float* dst = sino + static_cast<size_t>(view[v]) * D;
       const float* yrow = yres ? yres + static_cast<size_t>(view[v]) * D : nullptr;
       for (int b = threadIdx.x; b < D; b += 256) {
         float val = r[b];
-        for (int c = 1; c < 4 * FWD_CH; ++c) val = __fadd_rn(val, r[c * L + b]);
+        for (int c = 1; c < 4 * CH; ++c) val = __fadd_rn(val, r[c * L + b]);
         dst[b] = val;
         if (yrow) {
           const double d = static_cast<double>(__fsub_rn(yrow[b], val));
@@ -376,7 +444,7 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   }
 }
 
-template <int V>
+template <int V, int CH>
 __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
                                                   const float* __restrict__ imT,
                                                   float* __restrict__ sino,
@@ -388,34 +456,39 @@ __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
                                                   double* __restrict__ res_ws,
                                                   const int32_t* pred) {
   if (pred_off(pred)) return;
-  extern __shared__ float acc[];  // [V][4 waves][FWD_CH chains][L]
-  fwd_group<V>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws, blockIdx.x, acc);
+  extern __shared__ float acc[];  // [V][4 waves][CH chains][L]
+  fwd_group<V, CH>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws, blockIdx.x,
+                   acc);
 }
 
-// Adjoint (back projection).  One workgroup per ADJ_TI x ADJ_TJ pixel
-// tile; views are processed in chunks of ADJ_VC.  For each view the
+// Adjoint (back projection).  One workgroup per pixel tile of ADJ_TI rows
+// and ADJ_TJ * PPT columns (each thread owns PPT pixels of one row, ADJ_TJ
+// apart); views are processed in chunks of ADJ_VC.  For each view the
 // detector window reachable from the tile (W <= 64 bins, zero outside
 // [0, D)) and the view's geometry are staged in LDS, then every thread
-// gathers its pixel's two bins.  The windows of the next chunk are
+// gathers its pixels' two bins.  The windows of the next chunk are
 // prefetched into registers while the current chunk is computed, so the
 // global-load latency is hidden behind the gather arithmetic.  The two sums
 // of _xray2d.py:299-304 are kept separately and added at the end.
+// Px = (Pxmin + P0 i) + P1 j keeps the reference order; the row term
+// Pxmin + P0 i is shared by a thread's pixels.
 //
 // Window origin of view a for the tile at (i0, j0):
 //   lo = floor(P0 * i0 + P1 * j0 + clo[a]),
-//   clo[a] = Pxmin + min(P0, 0) (TI - 1) + min(P1, 0) (TJ - 1) - 1,
+//   clo[a] = Pxmin + min(P0, 0) (TI - 1) + min(P1, 0) (TJ PPT - 1) - 1,
 // a lower bound of floor(Px) over the tile with a one-bin margin for the
 // float32 rounding of Px; W covers the tile's projected extent plus the
 // margins, so every pixel's bins ind and ind + 1 lie inside the window.
 struct __align__(16) ViewLds {
-  float Pxmin, P0, P1, pad0;
+  float Pxmin, P0, P1;
+  int base4;  // byte offset in win[] of detector bin 0 for this view
   double rw;
-  int base;  // index in win[] of detector bin 0 for this view
-  int pad1;
+  double pad;
 };
 
 // One adjoint tile (tile row by, column bx; dot partial written at index
 // dot_i).
+template <int PPT>
 __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* __restrict__ out,
                                          const float4* __restrict__ geom,
                                          const double* __restrict__ rwt,
@@ -427,12 +500,23 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   __shared__ ViewLds gv[ADJ_VC];
   __shared__ double red[4];
   const int tx = threadIdx.x % ADJ_TJ, ty = threadIdx.x / ADJ_TJ;
-  const int i0 = by * ADJ_TI, j0 = bx * ADJ_TJ;
-  const int i = i0 + ty, j = j0 + tx;
-  const bool inside = (i < M) && (j < N);
-  const float fi = static_cast<float>(i), fj = static_cast<float>(j);
+  const int i0 = by * ADJ_TI, j0 = bx * (ADJ_TJ * PPT);
+  const int i = i0 + ty;
+  int j[PPT];
+  bool inside[PPT];
+  float fj[PPT];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    j[q] = j0 + tx + ADJ_TJ * q;
+    inside[q] = (i < M) && (j[q] < N);
+    fj[q] = static_cast<float>(j[q]);
+    any = any || inside[q];
+  }
+  const float fi = static_cast<float>(i);
   const float fi0 = static_cast<float>(i0), fj0 = static_cast<float>(j0);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const char* winb = reinterpret_cast<const char*>(win);
 
   float pf[VPW];
   int lop[VPW];
@@ -454,7 +538,9 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
     }
   };
 
-  float acc0 = 0.0f, acc1 = 0.0f;
+  float acc0[PPT], acc1[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) acc0[q] = acc1[q] = 0.0f;
   prefetch(0);
   for (int a0 = 0; a0 < A; a0 += ADJ_VC) {
     const int nv = min(ADJ_VC, A - a0);
@@ -474,56 +560,67 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
             t.P0 = g.y;
             t.P1 = g.z;
             t.rw = rwt[a0 + v];
-            t.base = v * W - lop[q];
+            t.base4 = 4 * (v * W - lop[q]);
           } else {
             t.Pxmin = t.P0 = t.P1 = 0.0f;
             t.rw = 1.0;
-            t.base = v * W;
+            t.base4 = 4 * v * W;
           }
-          t.pad0 = 0.0f;
-          t.pad1 = 0;
+          t.pad = 0.0;
           gv[v] = t;
         }
       }
     }
     __syncthreads();
     if (a0 + ADJ_VC < A) prefetch(a0 + ADJ_VC);
-    if (inside) {
+    if (any) {
       for (int v = 0; v < nv4; v += 4) {
-        float w[4];
-        int k[4];
+        float w[4][PPT];
+        int k[4][PPT];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const ViewLds t = gv[v + u];
-          const Weight wt = weight_at(t.Pxmin, t.P0, t.P1, 0.0f, t.rw, fi, fj);
-          w[u] = wt.w;
-          k[u] = static_cast<int>(wt.fl) + t.base;
-        }
-        float y0[4], y1[4];
+          const float ti = __fadd_rn(t.Pxmin, __fmul_rn(t.P0, fi));
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          y0[u] = win[k[u]];
-          y1[u] = win[k[u] + 1];
+          for (int q = 0; q < PPT; ++q) {
+            const float Px = __fadd_rn(ti, __fmul_rn(t.P1, fj[q]));
+            const float fl = floorf(Px);
+            const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
+            w[u][q] = fminf(div_rn(dd, t.rw), 1.0f);
+            k[u][q] = t.base4 + (static_cast<int>(fl) << 2);
+          }
         }
+        float y0[4][PPT], y1[4][PPT];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          acc0 = fmaf(y0[u], w[u], acc0);
-          acc1 = fmaf(y1[u], __fsub_rn(1.0f, w[u]), acc1);
-        }
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int q = 0; q < PPT; ++q) {
+            y0[u][q] = *reinterpret_cast<const float*>(winb + k[u][q]);
+            y1[u][q] = *reinterpret_cast<const float*>(winb + k[u][q] + 4);
+          }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int q = 0; q < PPT; ++q) {
+            acc0[q] = fmaf(y0[u][q], w[u][q], acc0[q]);
+            acc1[q] = fmaf(y1[u][q], __fsub_rn(1.0f, w[u][q]), acc1[q]);
+          }
       }
     }
   }
   double dot = 0.0;
-  if (inside) {
-    float v = __fadd_rn(acc0, acc1);
-    const bool circ = epi.circular != 0;
+  const bool circ = epi.circular != 0;
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    if (!inside[q]) continue;
+    float v = __fadd_rn(acc0[q], acc1[q]);
     if (epi.kind == EPI_GRAM) {  // (rho * C^T C)(p) + A^T A p   (_admmaux.py:222-229)
-      v = __fadd_rn(__fmul_rn(epi.rho, fd::gram2d(epi.p, M, N, i, j, circ)), v);
+      v = __fadd_rn(__fmul_rn(epi.rho, fd::gram2d(epi.p, M, N, i, j[q], circ)), v);
     } else if (epi.kind == EPI_RHS) {  // A^T y + rho * C^T (z - u)   (_admmaux.py:245-256)
-      v = __fadd_rn(v, __fmul_rn(epi.rho, fd::adj2d_diff(epi.z, epi.u, M, N, i, j, circ)));
+      v = __fadd_rn(v, __fmul_rn(epi.rho, fd::adj2d_diff(epi.z, epi.u, M, N, i, j[q], circ)));
     }
-    out[static_cast<size_t>(i) * N + j] = v;
-    if (epi.dot_ws) dot = static_cast<double>(epi.p[static_cast<size_t>(i) * N + j]) * v;
+    out[static_cast<size_t>(i) * N + j[q]] = v;
+    if (epi.dot_ws) dot += static_cast<double>(epi.p[static_cast<size_t>(i) * N + j[q]]) * v;
   }
   if (epi.dot_ws) {  // per-tile partial of sum p * out (CG curvature p^T A p)
     dot = block_sum_f64(dot, red);
@@ -531,6 +628,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   }
 }
 
+template <int PPT>
 __global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino,
                                                   float* __restrict__ out,
                                                   const float4* __restrict__ geom,
@@ -539,8 +637,8 @@ __global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino
                                                   int A, int D, int W, AdjEpi epi,
                                                   const int32_t* pred) {
   if (pred_off(pred)) return;
-  adj_tile(sino, out, geom, rwt, clo, M, N, A, D, W, epi, blockIdx.x, blockIdx.y,
-           blockIdx.y * gridDim.x + blockIdx.x);
+  adj_tile<PPT>(sino, out, geom, rwt, clo, M, N, A, D, W, epi, blockIdx.x, blockIdx.y,
+                blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // FBP row filter (_xray2d.py:187-194): the reference convolves each row
@@ -612,7 +710,7 @@ __global__ __launch_bounds__(256) void transpose_batch_kernel(BatchTabs t,
   }
 }
 
-template <int V>
+template <int V, int CH>
 __global__ __launch_bounds__(256) void fwd_batch_kernel(BatchTabs t, const float* __restrict__ im,
                                                         const float* __restrict__ imT,
                                                         float* __restrict__ sino,
@@ -627,13 +725,14 @@ __global__ __launch_bounds__(256) void fwd_batch_kernel(BatchTabs t, const float
     const int4 sh = t.shape[s];
     const int64_t io = t.ioff[s], so = t.soff[s];
     const int vo = t.voff[s], go = t.goff[s];
-    fwd_group<V>(im + io, imT + io, sino + so, t.geom + vo, t.rw + vo, t.groups + go,
+    fwd_group<V, CH>(im + io, imT + io, sino + so, t.geom + vo, t.rw + vo, t.groups + go,
                  t.order + vo, sh.x, sh.y, sh.w, yres ? yres + so : nullptr,
                  res_ws ? res_ws + go : nullptr, w.y, acc);
     __syncthreads();  // the next item re-zeroes acc
   }
 }
 
+template <int PPT>
 __global__ __launch_bounds__(256) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
                                                         float* __restrict__ out, BatchEpi be,
                                                         const int32_t* pred, int pstride) {
@@ -652,8 +751,8 @@ __global__ __launch_bounds__(256) void adj_batch_kernel(BatchTabs t, const float
     e.z = be.z ? be.z + 2 * io : nullptr;
     e.u = be.u ? be.u + 2 * io : nullptr;
     e.dot_ws = be.dot_ws ? be.dot_ws + t.toff[s] : nullptr;
-    adj_tile(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y, sh.z, sh.w,
-             me.x, e, w.y % me.z, w.y / me.z, w.y);
+    adj_tile<PPT>(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y, sh.z,
+                  sh.w, me.x, e, w.y % me.z, w.y / me.z, w.y);
     __syncthreads();
   }
 }
@@ -668,21 +767,26 @@ int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im
     SCXR_LAUNCHED("xr2d batch transpose");
   }
   const unsigned g = grid > 0 ? std::min(grid, t.nf) : t.nf;
-  const size_t lds = fwd_lds_bytes(V, Dmax);
-  switch (V) {
-    case 1:
-      hipLaunchKernelGGL(fwd_batch_kernel<1>, dim3(g), dim3(256), lds, s, t, im, imT, sino, yres,
-                         res_ws, pred, pstride);
-      break;
-    case 2:
-      hipLaunchKernelGGL(fwd_batch_kernel<2>, dim3(g), dim3(256), lds, s, t, im, imT, sino, yres,
-                         res_ws, pred, pstride);
-      break;
-    default:
-      hipLaunchKernelGGL(fwd_batch_kernel<4>, dim3(g), dim3(256), lds, s, t, im, imT, sino, yres,
-                         res_ws, pred, pstride);
-      break;
+  const size_t lds = fwd_lds_bytes(V, Dmax, t.ch);
+#define SCXR_FWDB(VV, CC)                                                                    \
+  hipLaunchKernelGGL((fwd_batch_kernel<VV, CC>), dim3(g), dim3(256), lds, s, t, im, imT, sino, \
+                     yres, res_ws, pred, pstride)
+  if (t.ch == 4) {
+    if (V == 1)
+      SCXR_FWDB(1, 4);
+    else if (V == 2)
+      SCXR_FWDB(2, 4);
+    else
+      SCXR_FWDB(4, 4);
+  } else {
+    if (V == 1)
+      SCXR_FWDB(1, 2);
+    else if (V == 2)
+      SCXR_FWDB(2, 2);
+    else
+      SCXR_FWDB(4, 2);
   }
+#undef SCXR_FWDB
   SCXR_LAUNCHED("xr2d batch forward");
   return SCXR_OK;
 }
@@ -690,8 +794,12 @@ int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im
 int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
                   const BatchEpi& epi, const int32_t* pred, int pstride, hipStream_t s) {
   const unsigned g = grid > 0 ? std::min(grid, t.na) : t.na;
-  hipLaunchKernelGGL(adj_batch_kernel, dim3(g), dim3(256), 0, s, t, sino, im, epi, pred,
-                     pstride);
+  if (t.ppt == 2)
+    hipLaunchKernelGGL(adj_batch_kernel<2>, dim3(g), dim3(256), 0, s, t, sino, im, epi, pred,
+                       pstride);
+  else
+    hipLaunchKernelGGL(adj_batch_kernel<1>, dim3(g), dim3(256), 0, s, t, sino, im, epi, pred,
+                       pstride);
   SCXR_LAUNCHED("xr2d batch adjoint");
   return SCXR_OK;
 }
@@ -703,30 +811,39 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
     hipLaunchKernelGGL(transpose_kernel, g, dim3(256), 0, s, im, p.work, p.M, p.N, pred);
     SCXR_LAUNCHED("xr2d transpose");
   }
-  const size_t lds = fwd_lds_bytes(p.vpg, p.D);
-  switch (p.vpg) {
-    case 1:
-      hipLaunchKernelGGL(fwd_kernel<1>, dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino,
-                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred);
-      break;
-    case 2:
-      hipLaunchKernelGGL(fwd_kernel<2>, dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino,
-                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred);
-      break;
-    default:
-      hipLaunchKernelGGL(fwd_kernel<4>, dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino,
-                         p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred);
-      break;
+  const size_t lds = fwd_lds_bytes(p.vpg, p.D, p.fwd_ch);
+#define SCXR_FWD(VV, CC)                                                                      \
+  hipLaunchKernelGGL((fwd_kernel<VV, CC>), dim3(p.n_groups), dim3(256), lds, s, im, p.work, sino, \
+                     p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, pred)
+  if (p.fwd_ch == 4) {
+    if (p.vpg == 1)
+      SCXR_FWD(1, 4);
+    else if (p.vpg == 2)
+      SCXR_FWD(2, 4);
+    else
+      SCXR_FWD(4, 4);
+  } else {
+    if (p.vpg == 1)
+      SCXR_FWD(1, 2);
+    else if (p.vpg == 2)
+      SCXR_FWD(2, 2);
+    else
+      SCXR_FWD(4, 2);
   }
+#undef SCXR_FWD
   SCXR_LAUNCHED("xr2d forward");
   return SCXR_OK;
 }
 
 int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,
             const AdjEpi& epi) {
-  dim3 g(ceil_div(p.N, ADJ_TJ), ceil_div(p.M, ADJ_TI));
-  hipLaunchKernelGGL(adj_kernel, g, dim3(256), 0, s, sino, im, p.geom, p.rw, p.clo, p.M, p.N,
-                     p.A, p.D, p.adj_w, epi, pred);
+  dim3 g(ceil_div(p.N, ADJ_TJ * p.adj_ppt), ceil_div(p.M, ADJ_TI));
+  if (p.adj_ppt == 2)
+    hipLaunchKernelGGL(adj_kernel<2>, g, dim3(256), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
+                       p.N, p.A, p.D, p.adj_w, epi, pred);
+  else
+    hipLaunchKernelGGL(adj_kernel<1>, g, dim3(256), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
+                       p.N, p.A, p.D, p.adj_w, epi, pred);
   SCXR_LAUNCHED("xr2d adjoint");
   return SCXR_OK;
 }
@@ -820,7 +937,7 @@ int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out) {
   SCXR_REQUIRE(plan && out, SCXR_ERR_ARG, "null argument to xr2d_info");
   const auto& p = plan->p;
   out[0] = p.n_groups;
-  out[1] = static_cast<int32_t>(ceil_div(p.N, ADJ_TJ) * ceil_div(p.M, ADJ_TI));
+  out[1] = static_cast<int32_t>(ceil_div(p.N, ADJ_TJ * p.adj_ppt) * ceil_div(p.M, ADJ_TI));
   out[2] = p.vpg;
   out[3] = p.adj_w;
   return SCXR_OK;

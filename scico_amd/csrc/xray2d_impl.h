@@ -9,27 +9,36 @@
 namespace scxr {
 namespace xr2d {
 
-// Adjoint tiling: ADJ_TI x ADJ_TJ pixels per workgroup (one per thread),
+// Adjoint tiling: ADJ_TI x (ADJ_TJ * PPT) pixels per workgroup (PPT per
+// thread, PPT = 1 or 2),
 // ADJ_VC views staged per LDS chunk, at most ADJ_WFAST bins per window.
 constexpr int ADJ_TI = 16;
 constexpr int ADJ_TJ = 16;
 constexpr int ADJ_VC = 32;
 constexpr int ADJ_WFAST = 64;  // window slots (one per lane)
 
-// Forward: 4 waves x FWD_CH chains private LDS copies of (D + 4) bins per
-// owned view.
-constexpr int FWD_CH = 2;
+// Forward: 4 waves x ch chains (4, or 2 for wide detectors) private LDS
+// copies of (D + 4) bins per owned view.
 constexpr size_t FWD_LDS_MAX = 64 * 1024;
 constexpr int FWD_SMAX = 64;  // largest lane stride of the forward projector
-inline size_t fwd_lds_bytes(int vpg, int D) {
-  return sizeof(float) * static_cast<size_t>(vpg) * 4 * FWD_CH * (static_cast<size_t>(D) + 4);
+inline size_t fwd_lds_bytes(int vpg, int D, int ch) {
+  return sizeof(float) * static_cast<size_t>(vpg) * 4 * ch * (static_cast<size_t>(D) + 4);
+}
+// Chains per wave for vpg views of D bins: 4 when the rows fit, else 2
+// (0: too wide).
+inline int fwd_chains(int vpg, int D) {
+  if (fwd_lds_bytes(vpg, D, 4) <= FWD_LDS_MAX) return 4;
+  if (fwd_lds_bytes(vpg, D, 2) <= FWD_LDS_MAX) return 2;
+  return 0;
 }
 
 struct Plan {
   int M = 0, N = 0, A = 0, D = 0;
   int vpg = 1;         // views per forward workgroup
+  int fwd_ch = 4;      // forward chains per wave
   int n_groups = 0;    // forward workgroups
   int adj_w = 0;       // adjoint window width (bins)
+  int adj_ppt = 1;     // adjoint pixels per thread
   bool any_colfast = false;
   void* dev_block = nullptr;  // device tables (one allocation)
   float4* geom = nullptr;     // (Pxmin, P0, P1, width) per view
@@ -64,11 +73,13 @@ struct HostTables {
   std::vector<int> order;
   std::vector<float> clo;
   int adj_w = 0;
+  int adj_ppt = 1;
   bool any_colfast = false;
 };
-// vpg: views per forward group.
+// vpg: views per forward group; ppt: adjoint pixels per thread (0 = 2 when
+// the adjoint window fits, else 1).
 int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
-                 double y0, const float* angles, int vpg);
+                 double y0, const float* angles, int vpg, int ppt);
 
 // Batched (ragged) slices: device tables of a batch plan.  Slice s has
 // shape (M, N, A, D) = shape[s]; its image starts at ioff[s] in packed
@@ -93,6 +104,8 @@ struct BatchTabs {
   const int2* aitems = nullptr;   // adjoint items (slice, tile)
   const int4* titems = nullptr;   // transpose items (slice, tile row, tile col, 0)
   int nf = 0, na = 0, nt = 0;
+  int ppt = 1;                    // adjoint pixels per thread (all slices)
+  int ch = 4;                     // forward chains per wave (all slices)
 };
 
 // Epilogue arguments of the batched adjoint (packed vectors; z, u are
