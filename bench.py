@@ -40,19 +40,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-VALU_PEAK = 256 * 64 * 2.4e9  # lane-instructions / s (256 CUs x 64 lanes x 2.4 GHz)
+# lane-instructions / s: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (a wave64 fp32,
+# fp64 or conversion instruction issues every 4 cycles on its SIMD)
+VALU_PEAK = 256 * 64 * 2.4e9
 
 # HBM traffic per launch of the 2D projector at 512^2 / 720 views / 724 bins
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled
 # by the counters of a 64 MiB copy (FETCH_SIZE reads 0.5x the bytes moved,
 # WRITE_SIZE 1.0x): forward = image transpose + projection.
 TRAFFIC_BYTES = {
-    "forward": 1024 * ((524.3 + 8468.5) * 2.0 + (1024.0 + 2047.5) * 1.0),
-    "adjoint": 1024 * (5112.0 * 2.0 + 1024.0 * 1.0),
+    "forward": 1024 * ((524.2 + 8456.5) * 2.0 + (1024.0 + 2047.5) * 1.0),
+    "adjoint": 1024 * (4498.5 * 2.0 + 1024.0 * 1.0),
 }
 # VALU instructions (lane-instructions) per (pixel, view) interaction,
 # rocprofv3 --pmc SQ_INSTS_VALU x 64 / interactions, same workload.
-VALU_PER_INTERACTION = {"forward": 22.63, "adjoint": 25.09}
+VALU_PER_INTERACTION = {"forward": 17.48, "adjoint": 19.34}
 
 
 def log(*a):
