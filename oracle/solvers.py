@@ -14,6 +14,8 @@ Restates in NumPy (float32 unless noted, reference operation order):
 * ADMM with ``LinearSubproblemSolver`` ``scico/optimize/_admm.py:96-378``,
   ``scico/optimize/_admmaux.py:160-269`` and its iteration statistics;
 * ``AcceleratedPGM`` ``scico/optimize/_pgm.py:60-252`` (constant step);
+* ``ProximalADMM`` ``scico/optimize/_padmm.py:171-361`` and the identity
+  ``SquaredL2Loss`` prox ``scico/loss.py:220-226``;
 * ``CircularConvolve`` ``scico/linop/_circconv.py:79-209``.
 
 Pins (tests/test_oracle_solvers.py): the reference's explicit FD outputs
@@ -22,7 +24,8 @@ the Haar identities ``W^T W = 2I / 4I`` (``test_tvnorm.py:13-23``), prox
 optimality against brute-force minimisation on tiny inputs, CG against a
 direct solve (``test_solver.py:28-114``), the ADMM / APGM fixed-point
 conditions and the quadratic closed forms (``test_admm.py:111-258``,
-``test_pgm.py:21-120``), CircularConvolve against a direct circular sum
+``test_pgm.py:21-120``), proximal ADMM converging to the closed-form
+minimiser of a quadratic problem (``test_padmm.py:180-200``), CircularConvolve against a direct circular sum
 (``test_circconv.py:35-56``) and ``numpy.fft`` in float64.
 """
 
@@ -343,6 +346,57 @@ def apgm(grad_f, prox_g, L0, x0, maxiter):
         t = 0.5 * (1 + np.sqrt(1 + 4 * t_old**2))
         v = (x + f32((t_old - 1) / t) * (x - x_old)).astype(f32)
     return x, res
+
+
+# -------------------------------------------------------- proximal ADMM --
+
+
+def _bl(op, *xs):
+    """Apply ``op`` blockwise to lists of arrays (a plain array is one block)."""
+    if isinstance(xs[0], list):
+        return [op(*t) for t in zip(*xs)]
+    return op(*xs)
+
+
+def padmm(f_prox, g_prox, A, AT, x0, z0, u0, rho, mu, nu, maxiter, B=None, BT=None, c=None,
+          dtype=f32):
+    """``ProximalADMM`` iterations (``scico/optimize/_padmm.py:348-361``).
+
+    ``z`` and ``u`` may be lists of arrays (block arrays); ``A`` returns and
+    ``AT`` takes such lists.  ``B = None`` means ``B = -I``; ``c = None``
+    means zero.  Returns ``(x, z, u)``.
+    """
+    cast = lambda a: np.asarray(a, dtype=dtype)
+    add = lambda a, b: _bl(lambda p, q: (p + q).astype(dtype), a, b)
+    sub = lambda a, b: _bl(lambda p, q: (p - q).astype(dtype), a, b)
+    scl = lambda s, a: _bl(lambda p: (dtype(s) * p).astype(dtype), a)
+    Bz = (lambda z: scl(-1.0, z)) if B is None else B
+    BH = (lambda w: scl(-1.0, w)) if BT is None else BT
+
+    def constraint(Ax, z):
+        r = add(Ax, Bz(z))
+        return r if c is None else sub(r, c)
+
+    x = cast(x0)
+    z = _bl(cast, z0)
+    u = _bl(cast, u0)
+    u_old = u
+    for _ in range(maxiter):
+        proxarg = sub(x, scl(1.0 / mu, AT(sub(scl(2.0, u), u_old))))
+        x = f_prox(proxarg, 1.0 / (rho * mu))
+        Ax = A(x)
+        proxarg = sub(z, scl(1.0 / nu, BH(add(constraint(Ax, z), u))))
+        z = g_prox(proxarg, 1.0 / (rho * nu))
+        u_old = u
+        u = add(u, constraint(Ax, z))
+    return x, z, u
+
+
+def sql2_identity_prox(v, lam, y, scale=0.5, dtype=f32):
+    """``SquaredL2Loss(y, A=Identity).prox`` (``loss.py:220-226``):
+    ``(c y + v) / (c + 1)`` with ``c = 2 scale lam``."""
+    c = dtype(2.0 * scale * lam)
+    return ((c * np.asarray(y, dtype) + np.asarray(v, dtype)) / (c + dtype(1))).astype(dtype)
 
 
 # ------------------------------------------------------- CircularConvolve --

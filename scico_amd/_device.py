@@ -37,8 +37,13 @@ def np_dtype(dt) -> np.dtype:
 
 
 def as_device(x, dtype=np.float32) -> torch.Tensor:
-    """Contiguous device tensor of ``dtype`` (copies host arrays to the GPU)."""
+    """Contiguous device tensor of ``dtype`` (copies host arrays to the GPU).
+    A block array becomes a block array of device tensors."""
     require_gpu()
+    from .blockarray import BlockArray
+
+    if isinstance(x, BlockArray):
+        return BlockArray(as_device(b, dtype) for b in x)
     td = torch_dtype(dtype)
     if isinstance(x, torch.Tensor):
         if x.device.type != "cuda":
@@ -71,12 +76,16 @@ def ptr(t: torch.Tensor | None):
 def to_numpy(x) -> np.ndarray:
     if isinstance(x, torch.Tensor):
         return x.detach().cpu().numpy()
+    if isinstance(x, list):
+        return [to_numpy(b) for b in x]
     return np.asarray(x)
 
 
 def array_dtype(x) -> np.dtype:
     if isinstance(x, torch.Tensor):
         return np_dtype(x.dtype)
+    if isinstance(x, list):
+        return array_dtype(x[0])
     return np.asarray(x).dtype
 
 

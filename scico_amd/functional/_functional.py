@@ -68,3 +68,35 @@ class ZeroFunctional(Functional):
 
     def prox(self, v, lam=1.0, **kwargs):
         return v
+
+
+class SeparableFunctional(Functional):
+    """``f(x) = sum_i f_i(x_i)`` over the blocks of a :class:`BlockArray`
+    (``scico/functional/_functional.py:201-270``); the prox is blockwise."""
+
+    def __init__(self, functional_list):
+        self.functional_list = list(functional_list)
+        self.has_eval = all(fi.has_eval for fi in self.functional_list)
+        self.has_prox = all(fi.has_prox for fi in self.functional_list)
+
+    def __call__(self, x):
+        if len(x) != len(self.functional_list):
+            raise ValueError(
+                f"Number of blocks in x, {len(x)}, and length of functional_list, "
+                f"{len(self.functional_list)}, do not match."
+            )
+        out = None
+        for fi, xi in zip(self.functional_list, x):
+            t = fi(xi)
+            out = t if out is None else out + t
+        return out
+
+    def prox(self, v, lam=1.0, **kwargs):
+        from ..blockarray import BlockArray
+
+        if len(v) != len(self.functional_list):
+            raise ValueError(
+                f"Number of blocks in v, {len(v)}, and length of functional_list, "
+                f"{len(self.functional_list)}, do not match."
+            )
+        return BlockArray(fi.prox(vi, lam, **kwargs) for fi, vi in zip(self.functional_list, v))

@@ -12,6 +12,19 @@ from .._lib import call
 from ._functional import Functional
 
 
+class SquaredL2Norm(Functional):
+    """``sum |x|^2``; prox ``v / (1 + 2 lam)`` (``_norm.py:119-158``)."""
+
+    has_eval = True
+    has_prox = True
+
+    def __call__(self, x):
+        return vec.reduce(vec.RED_SUMSQ, dev.as_device(x))
+
+    def prox(self, v, lam=1.0, **kwargs):
+        return vec.scale(1.0 / (1.0 + 2.0 * float(lam)), dev.as_device(v))
+
+
 class L1Norm(Functional):
     """``sum |x|``; prox is soft thresholding."""
 

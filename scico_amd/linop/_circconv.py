@@ -58,6 +58,9 @@ class CircularConvolve(LinearOperator):
         self.fft_shape = input_shape[-self.ndims:]
         self.batch = int(np.prod(input_shape[: len(input_shape) - self.ndims], dtype=np.int64))
         hh = h if isinstance(h, torch.Tensor) else np.asarray(h)
+        if hh.ndim > self.ndims and all(int(n) == 1 for n in hh.shape[: hh.ndim - self.ndims]):
+            # leading singleton axes broadcast over the batch axes
+            hh = hh.reshape(tuple(hh.shape[hh.ndim - self.ndims:]))
         if hh.ndim != self.ndims:
             raise NotImplementedError("Filter batch axes (multiple filters) are not supported.")
         self._h_is_dft = bool(h_is_dft)
@@ -135,7 +138,7 @@ class CircularConvolve(LinearOperator):
         return out
 
     @classmethod
-    def from_operator(cls, H, ndims=None, center=None, jit=True):
+    def from_operator(cls, H, ndims=None, center=None, device=None, jit=True):
         """``CircularConvolve`` from the impulse response of an LSI operator
         (``_circconv.py:258-307``)."""
         ndims = len(H.input_shape) if ndims is None else ndims

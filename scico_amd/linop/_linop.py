@@ -23,9 +23,10 @@ from .. import _vec as vec
 
 
 def _shape_tuple(s) -> tuple:
-    if isinstance(s, int):
-        return (s,)
-    return tuple(int(v) for v in s)
+    """A shape, or a tuple of block shapes (``BlockArray`` operands)."""
+    if isinstance(s, (int, np.integer)):
+        return (int(s),)
+    return tuple(_shape_tuple(v) if isinstance(v, (tuple, list)) else int(v) for v in s)
 
 
 def is_scalar(s) -> bool:

@@ -63,12 +63,17 @@ class SquaredL2Loss(Loss):
                               eval_fn=h, adj_fn=h, input_dtype=A.input_dtype)
 
     def prox(self, v, lam=1.0, **kwargs):
-        """``(I + lam 2 scale A^T A)^{-1} (v + lam 2 scale A^T y)`` by CG (``loss.py:211-243``)."""
+        """``(I + lam 2 scale A^T A)^{-1} (v + lam 2 scale A^T y)``: closed form for
+        the identity, CG otherwise (``loss.py:211-243``)."""
         from .solver import cg
 
         if not isinstance(self.A, LinearOperator):
             raise NotImplementedError("prox needs A to be a LinearOperator")
         v = dev.as_device(v)
+        if type(self.A) is Identity:
+            # diagonal A (``loss.py:220-226``): (c y + v) / (c + 1), c = 2 scale lam
+            c = 2.0 * self.scale * float(lam)
+            return vec.scale(1.0 / (c + 1.0), vec.axpby(c, self.y, 1.0, v))
         H = self.hessian
         c = float(lam)
         lhs = LinearOperator(input_shape=v.shape, output_shape=v.shape,

@@ -2,8 +2,10 @@
 
 from ._admm import ADMM
 from ._admmaux import GenericSubproblemSolver, LinearSubproblemSolver, SubproblemSolver
+from ._admmauxconv import CircularConvolveSolver
 from ._common import Optimizer
+from ._padmm import ProximalADMM, ProximalADMMBase
 from ._pgm import PGM, AcceleratedPGM, PGMStepSize
 
-__all__ = ["ADMM", "LinearSubproblemSolver", "GenericSubproblemSolver", "SubproblemSolver",
-           "Optimizer", "PGM", "AcceleratedPGM", "PGMStepSize"]
+__all__ = ["ADMM", "LinearSubproblemSolver", "CircularConvolveSolver", "GenericSubproblemSolver", "SubproblemSolver",
+           "Optimizer", "ProximalADMM", "ProximalADMMBase", "PGM", "AcceleratedPGM", "PGMStepSize"]

@@ -95,12 +95,21 @@ class Optimizer:
                 return np.stack([host(e) for e in v])
             return np.asarray(v)
 
+        from ..blockarray import BlockArray
+
+        state = {}
+        for k, v in self._get_state_variables().items():
+            if isinstance(v, BlockArray):  # blocks of different shapes: one entry each
+                state[f"{k}__nblocks"] = np.array(len(v))
+                for i, b in enumerate(v):
+                    state[f"{k}__b{i}"] = host(b)
+            else:
+                state[k] = host(v)
         hist = self.history(transpose=True)
         np.savez(path, opt_class=np.array(self.__class__.__name__), itnum=self.itnum,
                  history=np.array([list(c) for c in hist], dtype=np.float64) if len(hist[0]) else
                  np.zeros((len(hist), 0)),
-                 history_fields=np.array(list(hist._fields)),
-                 **{k: host(v) for k, v in self._get_state_variables().items()})
+                 history_fields=np.array(list(hist._fields)), **state)
 
     def load_state(self, path: str):
         """Restore a state saved by :meth:`save_state` (``_common.py:241-258``)."""
@@ -113,10 +122,18 @@ class Optimizer:
         obj = self.itstat_object
         obj._raw = [tuple(row) for row in hist.T] if hist.size else []
         obj.iterations = []
+        from ..blockarray import BlockArray
+
         state = {}
         for k in self._state_variable_names():
-            v = npz[k]
             cur = getattr(self, k)
+            if f"{k}__nblocks" in npz:
+                n = int(npz[f"{k}__nblocks"])
+                dv = cur[0].device if isinstance(cur, list) else cur.device
+                state[k] = BlockArray(torch.as_tensor(npz[f"{k}__b{i}"], device=dv)
+                                      for i in range(n))
+                continue
+            v = npz[k]
             if isinstance(cur, list):
                 state[k] = [torch.as_tensor(e, device=cur[0].device) for e in v]
             elif isinstance(cur, torch.Tensor):

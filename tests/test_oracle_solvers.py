@@ -192,3 +192,42 @@ def test_unrolled_apgm_oracle_matches_apgm():
     grad = lambda v: fAT((fA(v) - y).astype(np.float32))
     xr, _ = S.apgm(grad, lambda v, s: S.tv_prox(v, lam * s), L, np.zeros((8, 8), np.float32), 12)
     np.testing.assert_allclose(x, xr, rtol=1e-4, atol=1e-5)
+
+
+def test_padmm_oracle_quadratic_closed_form():
+    """Proximal ADMM (oracle) on argmin 1/2 ||y - M x||^2 + (lam/2) ||D x||^2
+    written as f = 0, g = (SquaredL2Loss(y), (lam/2) SquaredL2Norm),
+    A = (M; D), B = -I, converges to the solution of
+    (M^T M + lam D^T D) x = M^T y (the setting of ``test_padmm.py:180-200``)."""
+    rng = np.random.default_rng(12345)
+    M = rng.standard_normal((12, 8))
+    D = rng.standard_normal((10, 8))
+    y = rng.standard_normal(12)
+    lam = 1.0
+    A = lambda x: [M @ x, D @ x]
+    AT = lambda w: M.T @ w[0] + D.T @ w[1]
+    f_prox = lambda v, l: v
+    g_prox = lambda v, l: [S.sql2_identity_prox(v[0], l, y, dtype=np.float64),
+                           v[1] / (1.0 + l * lam)]
+    mu = 1.01 * np.linalg.norm(np.vstack([M, D]), 2) ** 2
+    x, z, u = S.padmm(f_prox, g_prox, A, AT, np.zeros(8), [np.zeros(12), np.zeros(10)],
+                      [np.zeros(12), np.zeros(10)], rho=1.0, mu=mu, nu=1.0, maxiter=4000,
+                      dtype=np.float64)
+    xs = np.linalg.solve(M.T @ M + lam * D.T @ D, M.T @ y)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+    # a sign slip in one block of A (adjoint left unchanged) breaks it
+    bad = lambda w: [-(M @ w), D @ w]
+    xb, _, _ = S.padmm(f_prox, g_prox, bad, AT, np.zeros(8), [np.zeros(12), np.zeros(10)],
+                       [np.zeros(12), np.zeros(10)], rho=1.0, mu=mu, nu=1.0, maxiter=400,
+                       dtype=np.float64)
+    assert np.linalg.norm(xb - xs) / np.linalg.norm(xs) > 1e-2
+
+
+def test_sql2_identity_prox_optimality():
+    """prox of 1/2 ||y - x||^2 at v minimises 1/2||x - v||^2 + lam/2 ||y - x||^2."""
+    rng = np.random.default_rng(3)
+    y, v = rng.standard_normal(50), rng.standard_normal(50)
+    lam = 0.7
+    p = S.sql2_identity_prox(v, lam, y, dtype=np.float64)
+    g = (p - v) + lam * (p - y)
+    assert np.abs(g).max() < 1e-12
