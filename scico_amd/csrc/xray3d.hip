@@ -55,6 +55,20 @@ struct Plan {
   float4* geom = nullptr;  // slice mode: (a, b, t, 0) in-plane row per view
   float* volc = nullptr;   // canonical volume workspace (P Q + 1 rows of 4 SQ; last row 0)
   float* proja = nullptr;  // canonical projection workspace (V U 4 SQ)
+  struct GenView* gview = nullptr;  // general mode: per-view gather set-up
+};
+
+// General-mode forward set-up of one view.  The detector pixel (d0, d1)
+// receives from the voxels whose projected centre P satisfies
+// floor(P_r - 1/4) in {d_r - 1, d_r}, i.e. P_r in [d_r - 3/4, d_r + 5/4).
+// The kernel marches along volume axis a (the one whose complementary
+// 2 x 2 minor of the matrix is largest) and, per plane, solves for the
+// other two coordinates (b, c) of the box centre with the inverse minor;
+// the candidates are the nb x nc integer points of the box's bounding
+// rectangle (half-extents hb, hc plus a rounding margin).
+struct GenView {
+  int a, nb, nc, pad;
+  float i00, i01, i10, i11, hb, hc, pad2, pad3;
 };
 
 // ------------------------------------------------------------ weights --
@@ -372,34 +386,100 @@ __global__ __launch_bounds__(256) void gen_adj_kernel(const float* __restrict__ 
   vol[e] = acc;
 }
 
+// General-mode forward as a gather: one thread per (view, detector pixel)
+// sums w * vol over the voxels whose reference taps land on the pixel, the
+// weights evaluated per voxel exactly as gen_weights does.  Deterministic,
+// no atomics.
+template <int A>
+__device__ __forceinline__ float gen_gather(const float* __restrict__ vol,
+                                            const float* __restrict__ m, const GenView& g,
+                                            int X, int Y, int Z, int d0, int d1, int t0,
+                                            int t1) {
+  constexpr int B = A == 0 ? 1 : 0, C = A == 2 ? 1 : 2;
+  const int dims[3] = {X, Y, Z};
+  const float r0 = __fsub_rn(__fadd_rn(static_cast<float>(d0), 0.25f), m[3]);
+  const float r1 = __fsub_rn(__fadd_rn(static_cast<float>(d1), 0.25f), m[7]);
+  float acc = 0.0f;
+  for (int t = t0; t < t1; ++t) {
+    const float xa = static_cast<float>(t) + 0.5f;
+    const float q0 = r0 - m[A] * xa, q1 = r1 - m[4 + A] * xa;
+    const float cb = g.i00 * q0 + g.i01 * q1, cc = g.i10 * q0 + g.i11 * q1;
+    const int lob = static_cast<int>(ceilf(cb - g.hb - 0.5f));
+    const int loc = static_cast<int>(ceilf(cc - g.hc - 0.5f));
+    for (int kb = 0; kb < g.nb; ++kb) {
+      const int ib = lob + kb;
+      if (ib < 0 || ib >= dims[B]) continue;
+      for (int kc = 0; kc < g.nc; ++kc) {
+        const int ic = loc + kc;
+        if (ic < 0 || ic >= dims[C]) continue;
+        const int i = A == 0 ? t : (B == 0 ? ib : ic);
+        const int j = A == 1 ? t : (B == 1 ? ib : ic);
+        const int k = A == 2 ? t : (C == 2 ? ic : ib);
+        const float x0 = __fadd_rn(static_cast<float>(i), 0.5f);
+        const float x1 = __fadd_rn(static_cast<float>(j), 0.5f);
+        const float x2 = __fadd_rn(static_cast<float>(k), 0.5f);
+        float tr[2];
+        int ul[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const float* mr = m + 4 * r;
+          const float P = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(mr[0], x0),
+                                                        __fmul_rn(mr[1], x1)),
+                                              __fmul_rn(mr[2], x2)),
+                                    mr[3]);
+          const float left = __fsub_rn(P, 0.25f);
+          tr[r] = fminf(__fsub_rn(ceilf(left), left), 0.5f);
+          ul[r] = static_cast<int>(floorf(left));
+        }
+        const bool h0 = ul[0] == d0, l0 = ul[0] == d0 - 1;
+        const bool h1 = ul[1] == d1, l1 = ul[1] == d1 - 1;
+        if ((h0 || l0) && (h1 || l1)) {
+          const float f0 = h0 ? tr[0] : __fsub_rn(0.5f, tr[0]);
+          const float f1 = h1 ? tr[1] : __fsub_rn(0.5f, tr[1]);
+          const float w = __fmul_rn(__fmul_rn(f0, f1), 4.0f);
+          acc = __builtin_fmaf(w, vol[(static_cast<size_t>(i) * Y + j) * Z + k], acc);
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// One workgroup per (view, 64 consecutive pixels): wave w takes planes
+// [w n / 4, (w + 1) n / 4) of the march axis and the four partial sums are
+// added in wave order.
 __global__ __launch_bounds__(256) void gen_fwd_kernel(const float* __restrict__ vol,
                                                       float* __restrict__ proj,
-                                                      const float* __restrict__ mats, int X,
+                                                      const float* __restrict__ mats,
+                                                      const GenView* __restrict__ gv, int X,
                                                       int Y, int Z, int D0, int D1,
                                                       const int32_t* __restrict__ pred) {
   if (pred_off(pred)) return;
-  const long long n = static_cast<long long>(X) * Y * Z;
-  const long long e = blockIdx.x * 256LL + threadIdx.x;
-  if (e >= n) return;
+  __shared__ float part[4][64];
   const int v = blockIdx.y;
-  const float x = vol[e];
-  if (x == 0.0f) return;
-  const int k = static_cast<int>(e % Z), j = static_cast<int>((e / Z) % Y),
-            i = static_cast<int>(e / (static_cast<long long>(Y) * Z));
-  int ul[2];
-  float w[4];
-  gen_weights(mats + 8 * v, i, j, k, D0, D1, ul, w);
-  float* y = proj + static_cast<size_t>(v) * D0 * D1;
-  if (w[0] != 0.0f) unsafeAtomicAdd(y + ul[0] * D1 + ul[1], __fmul_rn(w[0], x));
-  if (w[1] != 0.0f) unsafeAtomicAdd(y + (ul[0] + 1) * D1 + ul[1], __fmul_rn(w[1], x));
-  if (w[2] != 0.0f) unsafeAtomicAdd(y + ul[0] * D1 + ul[1] + 1, __fmul_rn(w[2], x));
-  if (w[3] != 0.0f) unsafeAtomicAdd(y + (ul[0] + 1) * D1 + ul[1] + 1, __fmul_rn(w[3], x));
-}
-
-__global__ __launch_bounds__(256) void zero_kernel(float* __restrict__ y, long long n,
-                                                   const int32_t* __restrict__ pred) {
-  if (pred_off(pred)) return;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += 256LL * gridDim.x) y[e] = 0.0f;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long e = blockIdx.x * 64LL + lane;
+  const bool ok = e < static_cast<long long>(D0) * D1;
+  const int d0 = ok ? static_cast<int>(e / D1) : 0, d1 = ok ? static_cast<int>(e % D1) : 0;
+  const float* m = mats + 8 * v;
+  const GenView g = gv[v];
+  const int n = g.a == 0 ? X : (g.a == 1 ? Y : Z);
+  const int t0 = (wave * n) >> 2, t1 = ((wave + 1) * n) >> 2;
+  float acc = 0.0f;
+  if (ok) {
+    if (g.a == 0)
+      acc = gen_gather<0>(vol, m, g, X, Y, Z, d0, d1, t0, t1);
+    else if (g.a == 1)
+      acc = gen_gather<1>(vol, m, g, X, Y, Z, d0, d1, t0, t1);
+    else
+      acc = gen_gather<2>(vol, m, g, X, Y, Z, d0, d1, t0, t1);
+  }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && ok)
+    proj[static_cast<size_t>(v) * D0 * D1 + e] =
+        __fadd_rn(__fadd_rn(__fadd_rn(part[0][lane], part[1][lane]), part[2][lane]),
+                  part[3][lane]);
 }
 
 // Reference weight arrays of one view: ul (2, X, Y, Z) int32 and
@@ -518,6 +598,8 @@ void destroy(Plan& p) {
   if (p.geom) (void)hipFree(p.geom);
   if (p.volc) (void)hipFree(p.volc);
   if (p.proja) (void)hipFree(p.proja);
+  if (p.gview) (void)hipFree(p.gview);
+  p.gview = nullptr;
   p.mats = nullptr;
   p.geom = nullptr;
   p.volc = p.proja = nullptr;
@@ -550,6 +632,44 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
   int rc = dev_alloc(reinterpret_cast<void**>(&p.mats), m.size() * sizeof(float));
   if (rc) return rc;
   SCXR_HIP(hipMemcpy(p.mats, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+  if (p.mode == MODE_GENERAL) {
+    std::vector<GenView> gv(V);
+    for (int v = 0; v < V; ++v) {
+      double best = 0.0;
+      int ba = -1;
+      for (int a = 0; a < 3; ++a) {
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        const double det = static_cast<double>(mat(m, v, 0, b)) * mat(m, v, 1, c) -
+                           static_cast<double>(mat(m, v, 0, c)) * mat(m, v, 1, b);
+        if (std::fabs(det) > std::fabs(best)) {
+          best = det;
+          ba = a;
+        }
+      }
+      SCXR_REQUIRE(ba >= 0 && std::fabs(best) > 1e-6, SCXR_ERR_GEOM,
+                   "projection matrix of view %d has rank < 2", v);
+      const int b = ba == 0 ? 1 : 0, c = ba == 2 ? 1 : 2;
+      GenView g{};
+      g.a = ba;
+      const double i00 = mat(m, v, 1, c) / best, i01 = -mat(m, v, 0, c) / best;
+      const double i10 = -mat(m, v, 1, b) / best, i11 = mat(m, v, 0, b) / best;
+      // box half-width 1 in P; margin for the float32 solve
+      const double hb = std::fabs(i00) + std::fabs(i01) + 1e-2;
+      const double hc = std::fabs(i10) + std::fabs(i11) + 1e-2;
+      g.i00 = static_cast<float>(i00);
+      g.i01 = static_cast<float>(i01);
+      g.i10 = static_cast<float>(i10);
+      g.i11 = static_cast<float>(i11);
+      g.hb = static_cast<float>(hb);
+      g.hc = static_cast<float>(hc);
+      g.nb = static_cast<int>(std::floor(2.0 * hb)) + 2;
+      g.nc = static_cast<int>(std::floor(2.0 * hc)) + 2;
+      gv[v] = g;
+    }
+    rc = dev_alloc(reinterpret_cast<void**>(&p.gview), V * sizeof(GenView));
+    if (rc) return rc;
+    SCXR_HIP(hipMemcpy(p.gview, gv.data(), V * sizeof(GenView), hipMemcpyHostToDevice));
+  }
   if (p.mode == MODE_SLICE) {
     const int e = 1 - p.dax;
     std::vector<float4> g(V);
@@ -658,13 +778,9 @@ static int canon_to_proj(const Plan& p, const float* yc, float* y, const int32_t
 
 int forward(const Plan& p, const float* vol, float* proj, const int32_t* pred, hipStream_t s) {
   if (p.mode == MODE_GENERAL) {
-    const long long n = static_cast<long long>(p.X) * p.Y * p.Z;
-    const long long ny = static_cast<long long>(p.V) * p.D0 * p.D1;
-    hipLaunchKernelGGL(zero_kernel, dim3(std::min<long long>((ny + 255) / 256, 65536)), dim3(256),
-                       0, s, proj, ny, pred);
-    SCXR_LAUNCHED("xr3d zero");
-    hipLaunchKernelGGL(gen_fwd_kernel, dim3(ceil_div(n, 256), p.V), dim3(256), 0, s, vol, proj,
-                       p.mats, p.X, p.Y, p.Z, p.D0, p.D1, pred);
+    const long long npix = static_cast<long long>(p.D0) * p.D1;
+    hipLaunchKernelGGL(gen_fwd_kernel, dim3(ceil_div(npix, 64), p.V), dim3(256), 0, s, vol,
+                       proj, p.mats, p.gview, p.X, p.Y, p.Z, p.D0, p.D1, pred);
     SCXR_LAUNCHED("xr3d general forward");
     return SCXR_OK;
   }
