@@ -251,6 +251,28 @@ def extra_3d(steps=5, warmup=2):
             "last_objective": float(hist[-1, 0])}
 
 
+def extra_3d_general(reps=10):
+    """The reference's own 3D projector comparison geometry
+    (``examples/scripts/ct_projector_comparison_3d.py:31-58``): a
+    129 x 130 x 131 volume, 3 views with Euler angles "XY" (74 deg, 0/60/120
+    deg), detector 128 x 129 -- not slice-decomposable, so the general-mode
+    gather kernels run."""
+    from scico_amd.linop.xray import XRayTransform3D
+
+    shape, det = (129, 130, 131), (128, 129)
+    ang = np.stack(np.broadcast_arrays(90.0 - 16.0, np.linspace(0, 180, 3, endpoint=False)),
+                   axis=-1)
+    M = XRayTransform3D.matrices_from_euler_angles(shape, det, "XY", ang, degrees=True)
+    H = XRayTransform3D(shape, M, det)
+    assert H.info()["mode"] == "general"
+    kt = time_kernels(H, reps=reps)
+    inter = float(np.prod(shape)) * 3
+    return {"workload": "xray3d_general_129x130x131_3views_XY74_det128x129",
+            "forward_ms": round(kt["forward"] * 1e3, 4),
+            "adjoint_ms": round(kt["adjoint"] * 1e3, 4),
+            "interactions_per_s": {k: round(inter / v, 1) for k, v in kt.items()}}
+
+
 def extra_batch(steps=3, warmup=1):
     """256 slices: N from {128, 192, 256, 384, 512}, 90..720 views,
     det_count = ceil(sqrt(2) N), ellipse phantoms (10) + 1 % noise;
@@ -459,6 +481,7 @@ def main():
     if not args.no_extras:
         extras = {}
         for name, fn in (("deconvolution", extra_deconv), ("xray3d", extra_3d),
+                         ("xray3d_general", extra_3d_general),
                          ("batched", extra_batch), ("unrolled_grad", extra_unrolled)):
             log(f"bench: extra {name}")
             t0 = time.perf_counter()
