@@ -145,3 +145,39 @@ def test_xray3d_single_axis_slices():
     p = X.project_3d(v, M, det)
     assert np.abs(p[:, :, [0, 1, 3, 4]]).max() == 0
     assert np.abs(p[:, :, 2]).max() > 0
+
+
+def _flip_rates(N, A, step=1):
+    """Fraction of (pixel, view) pairs whose detector bin changes when
+    (a) Px = (Pxmin + P0 i) + P1 j is evaluated with two fused multiply-adds
+    (one rounding each) and (b) cos/sin are float32 library results instead
+    of the correctly rounded float32 of the float64 values."""
+    g = X.xr2d_defaults((N, N), np.linspace(0, np.pi, A, endpoint=False))
+    views = np.arange(0, A, step)
+    th = g["angles"][views]
+    Pxmin, P0, P1, _ = X.view_params(g["x0"], g["dx"], th, g["y0"])
+    i = np.arange(N, dtype=np.float32).reshape(1, -1, 1)
+    j = np.arange(N, dtype=np.float32).reshape(1, 1, -1)
+    e = lambda v: v.reshape(-1, 1, 1)
+    ref = np.floor((e(Pxmin) + e(P0) * i) + e(P1) * j)
+    f64 = np.float64
+    t = (e(Pxmin).astype(f64) + e(P0).astype(f64) * i.astype(f64)).astype(np.float32)
+    fma = np.floor((t.astype(f64) + e(P1).astype(f64) * j.astype(f64)).astype(np.float32))
+    u0, u1 = np.cos(th), np.sin(th)  # float32 in, float32 library result out
+    x00, x01 = np.float32(g["x0"][0]), np.float32(g["x0"][1])
+    dx0, dx1 = np.float32(g["dx"][0]), np.float32(g["dx"][1])
+    Px0 = (x00 * u0 + x01 * u1) - g["y0"]
+    q0, q1 = dx0 * u0, dx1 * u1
+    c = np.stack([Px0, Px0 + q0, Px0 + q1, (Px0 + q0) + q1])
+    lib = np.floor((e(c.min(axis=0)) + e(q0) * i) + e(q1) * j)
+    n = ref.size
+    return float(np.count_nonzero(fma != ref)) / n, float(np.count_nonzero(lib != ref)) / n
+
+
+@pytest.mark.parametrize("N,A,step", [(256, 180, 1), (512, 720, 4)])
+def test_bin_flip_rates(N, A, step):
+    """Residual-risk measurement for the two XLA choices no fixture pins:
+    FMA contraction of the Px expression and float32 cos/sin rounding."""
+    r_fma, r_lib = _flip_rates(N, A, step)
+    print(f"\n  {N}^2/{A} views: bin flips with FMA {r_fma:.2e}, with float32 cos/sin {r_lib:.2e}")
+    assert r_fma < 1e-3 and r_lib < 1e-2
