@@ -487,7 +487,9 @@ struct __align__(16) ViewLds {
 };
 
 // One adjoint tile (tile row by, column bx; dot partial written at index
-// dot_i).
+// dot_i).  The workgroup is ADJ_VS groups of 256 threads over the same
+// pixels; group g takes the g-th contiguous share of the views, and the
+// groups' sums are added in group order at the end.
 template <int PPT>
 __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* __restrict__ out,
                                          const float4* __restrict__ geom,
@@ -496,10 +498,13 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
                                          int D, int W, const AdjEpi& epi, int bx, int by,
                                          int dot_i) {
   constexpr int VPW = ADJ_VC / 4;  // views staged per wave
-  __shared__ float win[ADJ_VC * ADJ_WFAST];
-  __shared__ ViewLds gv[ADJ_VC];
-  __shared__ double red[4];
-  const int tx = threadIdx.x % ADJ_TJ, ty = threadIdx.x / ADJ_TJ;
+  __shared__ float win_all[ADJ_VS][ADJ_VC * ADJ_WFAST];
+  __shared__ ViewLds gv_all[ADJ_VS][ADJ_VC];
+  __shared__ double red[4 * ADJ_VS];
+  const int grp = threadIdx.x >> 8, t = threadIdx.x & 255;
+  float* win = win_all[grp];
+  ViewLds* gv = gv_all[grp];
+  const int tx = t % ADJ_TJ, ty = t / ADJ_TJ;
   const int i0 = by * ADJ_TI, j0 = bx * (ADJ_TJ * PPT);
   const int i = i0 + ty;
   int j[PPT];
@@ -515,8 +520,12 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   }
   const float fi = static_cast<float>(i);
   const float fi0 = static_cast<float>(i0), fj0 = static_cast<float>(j0);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = t >> 6, lane = t & 63;
   const char* winb = reinterpret_cast<const char*>(win);
+  // this group's views [vb, ve); every group runs the same number of chunks
+  const int share = (A + ADJ_VS - 1) / ADJ_VS;
+  const int vb = min(A, grp * share), ve = min(A, vb + share);
+  const int nchunk = (share + ADJ_VC - 1) / ADJ_VC;
 
   float pf[VPW];
   int lop[VPW];
@@ -526,7 +535,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
       const int a = a0 + wave + 4 * q;
       float val = 0.0f;
       int lo = 0;
-      if (a < A) {
+      if (a < ve) {
         const float4 g = geom[a];
         lo = static_cast<int>(floorf(fmaf(g.y, fi0, fmaf(g.z, fj0, clo[a]))));
         const int b = lo + lane;
@@ -541,9 +550,10 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   float acc0[PPT], acc1[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) acc0[q] = acc1[q] = 0.0f;
-  prefetch(0);
-  for (int a0 = 0; a0 < A; a0 += ADJ_VC) {
-    const int nv = min(ADJ_VC, A - a0);
+  prefetch(vb);
+  for (int c = 0; c < nchunk; ++c) {
+    const int a0 = vb + c * ADJ_VC;
+    const int nv = max(0, min(ADJ_VC, ve - a0));
     const int nv4 = (nv + 3) & ~3;  // padded views: zero window, benign geometry
     __syncthreads();  // the previous chunk's gathers are done
 #pragma unroll
@@ -553,41 +563,41 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
         const bool real = v < nv;
         if (lane < W) win[v * W + lane] = real ? pf[q] : 0.0f;
         if (lane == 0) {
-          ViewLds t;
+          ViewLds tv;
           if (real) {
             const float4 g = geom[a0 + v];
-            t.Pxmin = g.x;
-            t.P0 = g.y;
-            t.P1 = g.z;
-            t.rw = rwt[a0 + v];
-            t.base4 = 4 * (v * W - lop[q]);
+            tv.Pxmin = g.x;
+            tv.P0 = g.y;
+            tv.P1 = g.z;
+            tv.rw = rwt[a0 + v];
+            tv.base4 = 4 * (v * W - lop[q]);
           } else {
-            t.Pxmin = t.P0 = t.P1 = 0.0f;
-            t.rw = 1.0;
-            t.base4 = 4 * v * W;
+            tv.Pxmin = tv.P0 = tv.P1 = 0.0f;
+            tv.rw = 1.0;
+            tv.base4 = 4 * v * W;
           }
-          t.pad = 0.0;
-          gv[v] = t;
+          tv.pad = 0.0;
+          gv[v] = tv;
         }
       }
     }
     __syncthreads();
-    if (a0 + ADJ_VC < A) prefetch(a0 + ADJ_VC);
+    if (a0 + ADJ_VC < ve) prefetch(a0 + ADJ_VC);
     if (any) {
       for (int v = 0; v < nv4; v += 4) {
         float w[4][PPT];
         int k[4][PPT];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const ViewLds t = gv[v + u];
-          const float ti = __fadd_rn(t.Pxmin, __fmul_rn(t.P0, fi));
+          const ViewLds tv = gv[v + u];
+          const float ti = __fadd_rn(tv.Pxmin, __fmul_rn(tv.P0, fi));
 #pragma unroll
           for (int q = 0; q < PPT; ++q) {
-            const float Px = __fadd_rn(ti, __fmul_rn(t.P1, fj[q]));
+            const float Px = __fadd_rn(ti, __fmul_rn(tv.P1, fj[q]));
             const float fl = floorf(Px);
             const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
-            w[u][q] = fminf(div_rn(dd, t.rw), 1.0f);
-            k[u][q] = t.base4 + (static_cast<int>(fl) << 2);
+            w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
+            k[u][q] = tv.base4 + (static_cast<int>(fl) << 2);
           }
         }
         float y0[4][PPT], y1[4][PPT];
@@ -608,28 +618,54 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
       }
     }
   }
+  // add the other groups' sums, in group order, into group 0
+  if (ADJ_VS > 1) {
+    float* xch = win_all[0];  // 2 PPT floats per thread of each group > 0
+    __syncthreads();
+    if (grp > 0) {
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        xch[((grp - 1) * 2 * PPT + 2 * q) * 256 + t] = acc0[q];
+        xch[((grp - 1) * 2 * PPT + 2 * q + 1) * 256 + t] = acc1[q];
+      }
+    }
+    __syncthreads();
+    if (grp == 0) {
+      for (int g2 = 1; g2 < ADJ_VS; ++g2)
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          acc0[q] = __fadd_rn(acc0[q], xch[((g2 - 1) * 2 * PPT + 2 * q) * 256 + t]);
+          acc1[q] = __fadd_rn(acc1[q], xch[((g2 - 1) * 2 * PPT + 2 * q + 1) * 256 + t]);
+        }
+    }
+  }
   double dot = 0.0;
   const bool circ = epi.circular != 0;
+  if (grp == 0) {
 #pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    if (!inside[q]) continue;
-    float v = __fadd_rn(acc0[q], acc1[q]);
-    if (epi.kind == EPI_GRAM) {  // (rho * C^T C)(p) + A^T A p   (_admmaux.py:222-229)
-      v = __fadd_rn(__fmul_rn(epi.rho, fd::gram2d(epi.p, M, N, i, j[q], circ)), v);
-    } else if (epi.kind == EPI_RHS) {  // A^T y + rho * C^T (z - u)   (_admmaux.py:245-256)
-      v = __fadd_rn(v, __fmul_rn(epi.rho, fd::adj2d_diff(epi.z, epi.u, M, N, i, j[q], circ)));
+    for (int q = 0; q < PPT; ++q) {
+      if (!inside[q]) continue;
+      float v = __fadd_rn(acc0[q], acc1[q]);
+      if (epi.kind == EPI_GRAM) {  // (rho * C^T C)(p) + A^T A p   (_admmaux.py:222-229)
+        v = __fadd_rn(__fmul_rn(epi.rho, fd::gram2d(epi.p, M, N, i, j[q], circ)), v);
+      } else if (epi.kind == EPI_RHS) {  // A^T y + rho * C^T (z - u)   (_admmaux.py:245-256)
+        v = __fadd_rn(v, __fmul_rn(epi.rho, fd::adj2d_diff(epi.z, epi.u, M, N, i, j[q], circ)));
+      }
+      out[static_cast<size_t>(i) * N + j[q]] = v;
+      if (epi.dot_ws) dot += static_cast<double>(epi.p[static_cast<size_t>(i) * N + j[q]]) * v;
     }
-    out[static_cast<size_t>(i) * N + j[q]] = v;
-    if (epi.dot_ws) dot += static_cast<double>(epi.p[static_cast<size_t>(i) * N + j[q]]) * v;
   }
   if (epi.dot_ws) {  // per-tile partial of sum p * out (CG curvature p^T A p)
-    dot = block_sum_f64(dot, red);
-    if (threadIdx.x == 0) epi.dot_ws[dot_i] = dot;
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) epi.dot_ws[dot_i] = ((red[0] + red[1]) + red[2]) + red[3];
   }
 }
 
 template <int PPT>
-__global__ __launch_bounds__(256) void adj_kernel(const float* __restrict__ sino,
+__global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4))) void adj_kernel(const float* __restrict__ sino,
                                                   float* __restrict__ out,
                                                   const float4* __restrict__ geom,
                                                   const double* __restrict__ rwt,
@@ -733,7 +769,7 @@ __global__ __launch_bounds__(256) void fwd_batch_kernel(BatchTabs t, const float
 }
 
 template <int PPT>
-__global__ __launch_bounds__(256) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
+__global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4))) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
                                                         float* __restrict__ out, BatchEpi be,
                                                         const int32_t* pred, int pstride) {
   for (int it = blockIdx.x; it < t.na; it += gridDim.x) {
@@ -795,10 +831,10 @@ int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
                   const BatchEpi& epi, const int32_t* pred, int pstride, hipStream_t s) {
   const unsigned g = grid > 0 ? std::min(grid, t.na) : t.na;
   if (t.ppt == 2)
-    hipLaunchKernelGGL(adj_batch_kernel<2>, dim3(g), dim3(256), 0, s, t, sino, im, epi, pred,
+    hipLaunchKernelGGL(adj_batch_kernel<2>, dim3(g), dim3(256 * ADJ_VS), 0, s, t, sino, im, epi, pred,
                        pstride);
   else
-    hipLaunchKernelGGL(adj_batch_kernel<1>, dim3(g), dim3(256), 0, s, t, sino, im, epi, pred,
+    hipLaunchKernelGGL(adj_batch_kernel<1>, dim3(g), dim3(256 * ADJ_VS), 0, s, t, sino, im, epi, pred,
                        pstride);
   SCXR_LAUNCHED("xr2d batch adjoint");
   return SCXR_OK;
@@ -839,10 +875,10 @@ int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hi
             const AdjEpi& epi) {
   dim3 g(ceil_div(p.N, ADJ_TJ * p.adj_ppt), ceil_div(p.M, ADJ_TI));
   if (p.adj_ppt == 2)
-    hipLaunchKernelGGL(adj_kernel<2>, g, dim3(256), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
+    hipLaunchKernelGGL(adj_kernel<2>, g, dim3(256 * ADJ_VS), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
                        p.N, p.A, p.D, p.adj_w, epi, pred);
   else
-    hipLaunchKernelGGL(adj_kernel<1>, g, dim3(256), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
+    hipLaunchKernelGGL(adj_kernel<1>, g, dim3(256 * ADJ_VS), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
                        p.N, p.A, p.D, p.adj_w, epi, pred);
   SCXR_LAUNCHED("xr2d adjoint");
   return SCXR_OK;
