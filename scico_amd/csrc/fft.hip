@@ -191,6 +191,164 @@ __global__ __launch_bounds__(256) void pass_kernel(const void* __restrict__ in,
   }
 }
 
+// Power-of-two lengths: Stockham auto-sort passes of radix 4 (and one of
+// radix 2 when log2 n is odd) on lpb lines held in one LDS buffer.  Each
+// thread takes whole butterflies: it reads its R inputs x[j + r n / R] into
+// registers, twiddles them by w^(k r) (k = j mod Ns, w = exp(-2 pi i /
+// (R Ns)), taken from the n-entry table), transforms them and, after a
+// barrier, writes them to x[(j / Ns) R Ns + k + r Ns]; the output is in
+// natural order, so the fused convolution pass multiplies by the spectrum
+// and runs the inverse stages on the same buffer.
+constexpr int SH_THREADS = 256;
+constexpr int SH_MAXV = 32;  // values per thread per stage (lpb n <= 8192)
+
+template <int R>
+__device__ __forceinline__ void dft_small(float2 (&v)[R], bool inv) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+    v[1] = make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y));
+  } else {
+    const float2 s02 = make_float2(__fadd_rn(v[0].x, v[2].x), __fadd_rn(v[0].y, v[2].y));
+    const float2 d02 = make_float2(__fsub_rn(v[0].x, v[2].x), __fsub_rn(v[0].y, v[2].y));
+    const float2 s13 = make_float2(__fadd_rn(v[1].x, v[3].x), __fadd_rn(v[1].y, v[3].y));
+    const float2 d13 = make_float2(__fsub_rn(v[1].x, v[3].x), __fsub_rn(v[1].y, v[3].y));
+    // -i d13 (forward) or +i d13 (inverse)
+    const float2 t = inv ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+    v[0] = make_float2(__fadd_rn(s02.x, s13.x), __fadd_rn(s02.y, s13.y));
+    v[2] = make_float2(__fsub_rn(s02.x, s13.x), __fsub_rn(s02.y, s13.y));
+    v[1] = make_float2(__fadd_rn(d02.x, t.x), __fadd_rn(d02.y, t.y));
+    v[3] = make_float2(__fsub_rn(d02.x, t.x), __fsub_rn(d02.y, t.y));
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void stockham_stage(float2* X, int n, int lpb, int Ns,
+                                               const float2* __restrict__ tw, bool inv) {
+  const int nb = n / R, tot = lpb * nb;  // butterflies
+  const int tstep = n / (R * Ns);
+  float2 v[SH_MAXV / R][R];
+  int dst[SH_MAXV / R];
+#pragma unroll
+  for (int q = 0; q < SH_MAXV / R; ++q) {
+    const int b = threadIdx.x + q * SH_THREADS;
+    dst[q] = -1;
+    if (b < tot) {
+      const int l = b / nb, j = b - l * nb;
+      float2* x = X + l * n;
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = x[j + r * nb];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[(k * r * tstep) & (n - 1)];
+        if (inv) w = conjf2(w);
+        v[q][r] = cmul(v[q][r], w);
+      }
+      dft_small<R>(v[q], inv);
+      dst[q] = l * n + (j - k) * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SH_MAXV / R; ++q) {
+    if (dst[q] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) X[dst[q] + r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void stockham(float2* X, int n, int log2n, int lpb,
+                                         const float2* __restrict__ tw, bool inv) {
+  int Ns = 1;
+  if (log2n & 1) {
+    stockham_stage<2>(X, n, lpb, Ns, tw, inv);
+    Ns = 2;
+  }
+  for (; Ns < n; Ns *= 4) stockham_stage<4>(X, n, lpb, Ns, tw, inv);
+}
+
+__global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __restrict__ in,
+                                                              void* __restrict__ out, Pass P,
+                                                              float scale,
+                                                              const int32_t* pred) {
+  if (pred_off(pred)) return;
+  extern __shared__ float2 lds[];  // lpb * n
+  __shared__ long long base[64];
+  const int n = P.n, lpb = P.lpb, tot = lpb * n;
+  float2* X = lds;
+  const bool contiguous = P.stride == 1;
+  // Strided passes: workgroups dispatched to one XCD (consecutive ids go to
+  // the 8 XCDs in turn) take consecutive line groups, so the cache lines
+  // that neighbouring columns share stay in that XCD's L2.
+  long long blk = blockIdx.x;
+  if (!contiguous) {
+    const long long G = gridDim.x, per = (G + 7) / 8;
+    const long long x = blk % 8, q = blk / 8;
+    const long long cand = x * per + q;
+    // the last XCD's share may be short: keep the identity map when the
+    // remapped index would fall outside the grid
+    blk = (G % 8 == 0) ? cand : blk;
+  }
+  const long long line0 = blk * lpb;
+  const int nl = static_cast<int>(min(static_cast<long long>(lpb), P.lines - line0));
+  if (threadIdx.x < lpb) {
+    const long long L = line0 + threadIdx.x;
+    const long long outer = L / P.inner, inr = L - outer * P.inner;
+    base[threadIdx.x] = outer * n * P.stride + inr;
+  }
+  __syncthreads();
+  auto split = [&](int e, int& l, int& k) {
+    if (contiguous) {
+      l = e / n;
+      k = e - l * n;
+    } else {
+      k = e / lpb;
+      l = e - k * lpb;
+    }
+  };
+  for (int e = threadIdx.x; e < tot; e += SH_THREADS) {
+    int l, k;
+    split(e, l, k);
+    float2 v = make_float2(0.0f, 0.0f);
+    if (l < nl) {
+      const long long g = base[l] + static_cast<long long>(k) * P.stride;
+      v = P.in_real ? make_float2(static_cast<const float*>(in)[g], 0.0f)
+                    : static_cast<const float2*>(in)[g];
+    }
+    X[l * n + k] = v;
+  }
+  __syncthreads();
+  stockham(X, n, P.log2n, lpb, P.tw, P.fused ? false : (P.inverse != 0));
+  if (P.fused) {
+    for (int e = threadIdx.x; e < tot; e += SH_THREADS) {
+      const int l = e / n, k = e - l * n;
+      if (l < nl) {
+        const long long inr = (line0 + l) % P.inner;
+        float2 hv = P.h[static_cast<long long>(k) * P.stride + inr];
+        if (P.conj_h) hv = conjf2(hv);
+        X[e] = cmul(hv, X[e]);
+      }
+    }
+    __syncthreads();
+    stockham(X, n, P.log2n, lpb, P.tw, true);
+  }
+  for (int e = threadIdx.x; e < tot; e += SH_THREADS) {
+    int l, k;
+    split(e, l, k);
+    if (l < nl) {
+      const long long g = base[l] + static_cast<long long>(k) * P.stride;
+      const float2 v = X[l * n + k];
+      if (P.out_real)
+        static_cast<float*>(out)[g] = __fmul_rn(v.x, scale);
+      else
+        static_cast<float2*>(out)[g] = make_float2(__fmul_rn(v.x, scale), __fmul_rn(v.y, scale));
+    }
+  }
+}
+
 // Elementwise complex product c = a * b (spectrum centring shift).
 __global__ __launch_bounds__(256) void cmul_kernel(long long n, const float2* __restrict__ a,
                                                    const float2* __restrict__ b,
@@ -229,7 +387,7 @@ int make_tables(AxisTables& t, int n) {
   SCXR_REQUIRE(t.log2n >= 0 ? n <= MAX_POW2 : n <= MAX_DFT, SCXR_ERR_UNSUPPORTED,
                "FFT length %d unsupported (powers of two up to %d, other lengths up to %d)", n,
                MAX_POW2, MAX_DFT);
-  const int cnt = t.log2n >= 0 ? std::max(1, n / 2) : n;
+  const int cnt = n;  // exp(-2 pi i k / n), k < n
   std::vector<float2> tw(cnt);
   for (int k = 0; k < cnt; ++k) {
     const double a = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
@@ -260,6 +418,29 @@ int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool
   P.stride = stride;
   P.inner = stride;
   P.lines = p.batch * (p.fsize / t.n);
+  if (t.log2n >= 2) {
+    // Stockham: one LDS buffer of lpb lines (at most 8192 values); lines per
+    // workgroup chosen so that a pass has at least ~1024 workgroups.
+    const long long lines = p.batch * (p.fsize / t.n);
+    int lpb = std::max(1, 8192 / t.n);
+    while (lpb > 1 && (lines + lpb - 1) / lpb < 1024) lpb /= 2;
+    lpb = std::min(lpb, 64);
+    P.lpb = lpb;
+    P.in_real = in_real;
+    P.out_real = out_real;
+    P.inverse = inverse;
+    P.fused = fused;
+    P.conj_h = conj_h;
+    P.tw = t.tw;
+    P.h = p.h;
+    const float scale = (inverse || fused) ? 1.0f / static_cast<float>(t.n) : 1.0f;
+    const size_t lds = sizeof(float2) * lpb * t.n;
+    const unsigned grid = static_cast<unsigned>((P.lines + lpb - 1) / lpb);
+    hipLaunchKernelGGL(stockham_kernel, dim3(grid), dim3(SH_THREADS), lds, s, in, out, P, scale,
+                       pred);
+    SCXR_LAUNCHED("fft stockham pass");
+    return SCXR_OK;
+  }
   // Two LDS buffers for the fused and direct-DFT passes, one otherwise;
   // at most 64 KiB of LDS per workgroup in total.
   const bool two = fused || t.log2n < 0;
