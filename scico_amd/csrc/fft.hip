@@ -224,7 +224,7 @@ __device__ __forceinline__ void dft_small(float2 (&v)[R], bool inv) {
 
 template <int R>
 __device__ __forceinline__ void stockham_stage(float2* X, int n, int lpb, int Ns,
-                                               const float2* __restrict__ tw, bool inv) {
+                                               const float2* tw, bool inv) {
   const int nb = n / R, tot = lpb * nb;  // butterflies
   const int tstep = n / (R * Ns);
   float2 v[SH_MAXV / R][R];
@@ -261,7 +261,7 @@ __device__ __forceinline__ void stockham_stage(float2* X, int n, int lpb, int Ns
 }
 
 __device__ __forceinline__ void stockham(float2* X, int n, int log2n, int lpb,
-                                         const float2* __restrict__ tw, bool inv) {
+                                         const float2* tw, bool inv) {
   int Ns = 1;
   if (log2n & 1) {
     stockham_stage<2>(X, n, lpb, Ns, tw, inv);
@@ -275,10 +275,25 @@ __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __rest
                                                               float scale,
                                                               const int32_t* pred) {
   if (pred_off(pred)) return;
-  extern __shared__ float2 lds[];  // lpb * n
+  extern __shared__ float2 lds[];  // lpb * n values, then the n twiddles
   __shared__ long long base[64];
+  __shared__ long long hoff[64];  // spectrum offset of each line (fused pass)
   const int n = P.n, lpb = P.lpb, tot = lpb * n;
   float2* X = lds;
+  float2* twl = lds + tot;
+  {
+    float2 tv[SH_MAXV / 2];  // n <= 4096: at most 16 per thread, all loads issued first
+#pragma unroll
+    for (int q = 0; q < SH_MAXV / 2; ++q) {
+      const int e = threadIdx.x + q * SH_THREADS;
+      if (e < n) tv[q] = P.tw[e];
+    }
+#pragma unroll
+    for (int q = 0; q < SH_MAXV / 2; ++q) {
+      const int e = threadIdx.x + q * SH_THREADS;
+      if (e < n) twl[e] = tv[q];
+    }
+  }
   const bool contiguous = P.stride == 1;
   // Strided passes: workgroups dispatched to one XCD (consecutive ids go to
   // the 8 XCDs in turn) take consecutive line groups, so the cache lines
@@ -298,6 +313,7 @@ __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __rest
     const long long L = line0 + threadIdx.x;
     const long long outer = L / P.inner, inr = L - outer * P.inner;
     base[threadIdx.x] = outer * n * P.stride + inr;
+    hoff[threadIdx.x] = inr;
   }
   __syncthreads();
   auto split = [&](int e, int& l, int& k) {
@@ -309,31 +325,47 @@ __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __rest
       l = e - k * lpb;
     }
   };
-  for (int e = threadIdx.x; e < tot; e += SH_THREADS) {
-    int l, k;
-    split(e, l, k);
-    float2 v = make_float2(0.0f, 0.0f);
-    if (l < nl) {
-      const long long g = base[l] + static_cast<long long>(k) * P.stride;
-      v = P.in_real ? make_float2(static_cast<const float*>(in)[g], 0.0f)
-                    : static_cast<const float2*>(in)[g];
+  {
+    // every load of the thread is issued before the first LDS store
+    float2 vv[SH_MAXV];
+#pragma unroll
+    for (int q = 0; q < SH_MAXV; ++q) {
+      const int e = threadIdx.x + q * SH_THREADS;
+      if (e < tot) {
+        int l, k;
+        split(e, l, k);
+        float2 v = make_float2(0.0f, 0.0f);
+        if (l < nl) {
+          const long long g = base[l] + static_cast<long long>(k) * P.stride;
+          v = P.in_real ? make_float2(static_cast<const float*>(in)[g], 0.0f)
+                        : static_cast<const float2*>(in)[g];
+        }
+        vv[q] = v;
+      }
     }
-    X[l * n + k] = v;
+#pragma unroll
+    for (int q = 0; q < SH_MAXV; ++q) {
+      const int e = threadIdx.x + q * SH_THREADS;
+      if (e < tot) {
+        int l, k;
+        split(e, l, k);
+        X[l * n + k] = vv[q];
+      }
+    }
   }
   __syncthreads();
-  stockham(X, n, P.log2n, lpb, P.tw, P.fused ? false : (P.inverse != 0));
+  stockham(X, n, P.log2n, lpb, twl, P.fused ? false : (P.inverse != 0));
   if (P.fused) {
     for (int e = threadIdx.x; e < tot; e += SH_THREADS) {
       const int l = e / n, k = e - l * n;
       if (l < nl) {
-        const long long inr = (line0 + l) % P.inner;
-        float2 hv = P.h[static_cast<long long>(k) * P.stride + inr];
+        float2 hv = P.h[static_cast<long long>(k) * P.stride + hoff[l]];
         if (P.conj_h) hv = conjf2(hv);
         X[e] = cmul(hv, X[e]);
       }
     }
     __syncthreads();
-    stockham(X, n, P.log2n, lpb, P.tw, true);
+    stockham(X, n, P.log2n, lpb, twl, true);
   }
   for (int e = threadIdx.x; e < tot; e += SH_THREADS) {
     int l, k;
@@ -422,7 +454,8 @@ int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool
     // Stockham: one LDS buffer of lpb lines (at most 8192 values); lines per
     // workgroup chosen so that a pass has at least ~1024 workgroups.
     const long long lines = p.batch * (p.fsize / t.n);
-    int lpb = std::max(1, 8192 / t.n);
+    int lpb = std::max(1, 8192 / t.n - 1);  // plus one line's worth of twiddles
+    { int q = 1; while (q * 2 <= lpb) q *= 2; lpb = q; }
     while (lpb > 1 && (lines + lpb - 1) / lpb < 1024) lpb /= 2;
     lpb = std::min(lpb, 64);
     P.lpb = lpb;
@@ -434,7 +467,7 @@ int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool
     P.tw = t.tw;
     P.h = p.h;
     const float scale = (inverse || fused) ? 1.0f / static_cast<float>(t.n) : 1.0f;
-    const size_t lds = sizeof(float2) * lpb * t.n;
+    const size_t lds = sizeof(float2) * (lpb + 1) * t.n;
     const unsigned grid = static_cast<unsigned>((P.lines + lpb - 1) / lpb);
     hipLaunchKernelGGL(stockham_kernel, dim3(grid), dim3(SH_THREADS), lds, s, in, out, P, scale,
                        pred);
