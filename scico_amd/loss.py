@@ -43,11 +43,40 @@ class SquaredL2Loss(Loss):
         return vec.reduce(vec.RED_SQDIST, self.y, self.A(x), scale_=self.scale)
 
     def grad(self, x):
-        """``2 scale A^T (A x - y)`` (the autodiff gradient of ``__call__``)."""
-        r = vec.add(self.A(x), -1.0, self.y)
-        g = self.A.adj(r)
+        """``2 scale A^T (A x - y)`` (the autodiff gradient of ``__call__``).
+
+        For a :class:`CircularConvolve` the gradient is evaluated as
+        ``2 scale (G x - A^T y)`` with ``G = A^T A`` the circular
+        convolution by ``|H|^2`` and ``A^T y`` cached: one FFT convolution
+        per gradient instead of two (equal up to float32 rounding).
+        """
+        from .linop import CircularConvolve
+
         c = 2.0 * self.scale
+        if type(self.A) is CircularConvolve:
+            G, ATy = self._conv_gram()
+            g = vec.add(G(x), -1.0, ATy)
+        else:
+            r = vec.add(self.A(x), -1.0, self.y)
+            g = self.A.adj(r)
         return g if c == 1.0 else vec.scale(c, g)
+
+    def _conv_gram(self):
+        """Cached (G, A^T y) for a circular-convolution operator; rebuilt
+        when the operator or y changes."""
+        from .linop import CircularConvolve
+
+        key = (id(self.A), self.y.data_ptr(), self.y._version)
+        cache = getattr(self, "_gram_cache", None)
+        if cache is None or cache[0] != key:
+            A = self.A
+            hd = dev.to_numpy(A.h_dft).astype(np.complex64)
+            g = (hd.real.astype(np.float64) ** 2 + hd.imag.astype(np.float64) ** 2)
+            G = CircularConvolve(g.astype(np.complex64), A.input_shape, ndims=A.ndims,
+                                 h_is_dft=True)
+            cache = (key, G, A.adj(self.y))
+            self._gram_cache = cache
+        return cache[1], cache[2]
 
     @property
     def hessian(self) -> LinearOperator:
