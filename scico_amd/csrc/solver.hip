@@ -634,6 +634,23 @@ __device__ __forceinline__ void tv_coeffs(const float* __restrict__ v, const TvD
   }
 }
 
+// Pixel coordinates of linear index e (32-bit arithmetic when the volume
+// has fewer than 2^32 pixels).
+__device__ __forceinline__ void tv_split(int64_t e, const TvDims& t, bool small, int (&p)[3]) {
+  if (small) {
+    const unsigned u = static_cast<unsigned>(e), d2 = static_cast<unsigned>(t.d[2]);
+    const unsigned d1 = static_cast<unsigned>(t.d[1]);
+    const unsigned q = u / d2;
+    p[2] = static_cast<int>(u - q * d2);
+    p[1] = static_cast<int>(q % d1);
+    p[0] = static_cast<int>(q / d1);
+  } else {
+    p[0] = static_cast<int>(e / (static_cast<int64_t>(t.d[1]) * t.d[2]));
+    p[1] = static_cast<int>((e / t.d[2]) % t.d[1]);
+    p[2] = static_cast<int>(e % t.d[2]);
+  }
+}
+
 __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ v,
                                                       float* __restrict__ out, TvDims t,
                                                       float tau, float invK, int iso,
@@ -643,8 +660,8 @@ __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ 
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
   const float s = 0.70710677f;
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
-    const int p[3] = {static_cast<int>(e / (static_cast<int64_t>(t.d[1]) * t.d[2])),
-                      static_cast<int>((e / t.d[2]) % t.d[1]), static_cast<int>(e % t.d[2])};
+    int p[3];
+    tv_split(e, t, n <= 0xffffffffll, p);
     float Hp[3], Lp[3];
     tv_coeffs(v, t, p, tau, iso, w, Hp, Lp);
     float acc = 0.0f;
@@ -739,8 +756,8 @@ __global__ __launch_bounds__(256) void tv_vjp_v_kernel(const float* __restrict__
   const int64_t np = static_cast<int64_t>(t.P[0]) * t.P[1] * t.P[2];
   const float s = 0.70710677f;
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
-    const int p[3] = {static_cast<int>(e / (static_cast<int64_t>(t.d[1]) * t.d[2])),
-                      static_cast<int>((e / t.d[2]) % t.d[1]), static_cast<int>(e % t.d[2])};
+    int p[3];
+    tv_split(e, t, n <= 0xffffffffll, p);
     // padded locations replicating p: q_a in {p_a} or {p_a, d_a} on the last index
     int nalt[3];
     for (int a = 0; a < 3; ++a) nalt[a] = (!t.circ && a >= t.first && p[a] == t.d[a] - 1) ? 2 : 1;
@@ -775,8 +792,8 @@ __global__ __launch_bounds__(256) void tv_norm_kernel(const float* __restrict__ 
   const int64_t st[3] = {static_cast<int64_t>(t.d[1]) * t.d[2], t.d[2], 1};
   double acc = 0.0;
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += 256ll * gridDim.x) {
-    const int p[3] = {static_cast<int>(e / st[0]), static_cast<int>((e / t.d[2]) % t.d[1]),
-                      static_cast<int>(e % t.d[2])};
+    int p[3];
+    tv_split(e, t, n <= 0xffffffffll, p);
     const float xc = x[e];
     double ss = 0.0;
     for (int a = t.first; a < 3; ++a) {
