@@ -289,15 +289,29 @@ __global__ __launch_bounds__(256) void adj_slice_kernel(const float4* __restrict
     const int nv = min(64 / AR, V - v0);
     for (int j = 0; j < nv; ++j) {
       const int vb = (v0 + j) * U;
+      // The bin pair (u, u + 1) is uniform across the wave; consecutive
+      // pixels of a row move it by at most one bin, so rows already loaded
+      // for the previous pixel are reused (scalar branches).
+      auto row = [&](int u) { return bload4(yr, qoff, (vb + min(max(u, 0), U - 1)) * rowb); };
+      int ucur = INT_MIN / 2;
+      float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
 #pragma unroll
       for (int i = 0; i < AR; ++i) {
         const int L = j * AR + i;
         const int u = rl(ul, L);
         const float cl = rlf(wl, L), cr = rlf(wr, L);
-        const int lo = min(max(u, 0), U - 1), hi = min(max(u + 1, 0), U - 1);
-        const float4 ylo = bload4(yr, qoff, (vb + lo) * rowb);
-        const float4 yhi = bload4(yr, qoff, (vb + hi) * rowb);
-        acc[i] = fma4(cr, yhi, fma4(cl, ylo, acc[i]));
+        if (u == ucur + 1) {
+          ya = yb;
+          yb = row(u + 1);
+        } else if (u == ucur - 1) {
+          yb = ya;
+          ya = row(u);
+        } else if (u != ucur) {
+          ya = row(u);
+          yb = row(u + 1);
+        }
+        ucur = u;
+        acc[i] = fma4(cr, yb, fma4(cl, ya, acc[i]));
       }
     }
   }
