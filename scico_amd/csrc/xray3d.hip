@@ -166,8 +166,8 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
   const unsigned quad = blockIdx.z * 64 + lane;
   const bool qok = quad < static_cast<unsigned>(SQ);
   const unsigned qoff = (qok ? quad : 0u) * 16u;
-  // vol holds P Q rows of SQ slice quads plus one zero row (read by empty
-  // slots)
+  // vol holds P Q rows of SQ slice quads plus one zero row; the buffer
+  // bound ends after that row
   const int rowb = SQ * 16;
   const int zoff = P * Q * rowb;
   const __amdgpu_buffer_rsrc_t vr = make_rsrc(vol, zoff + rowb);
@@ -230,12 +230,14 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
       for (int gg = -1; gg < FB; ++gg) {
 #pragma unroll
         for (int k = 0; k < SPG; ++k) {
-          // Empty slots read the zero row with the weights of lane
-          // base + CPR - 1, which never holds a candidate (zero weights).
+          // Empty slots use the weights of lane base + CPR - 1, which never
+          // holds a candidate (zero weights).
           const int L = first[gg + 1] + k;
           const bool ok = L < first[gg + 2];
           const int Lc = ok ? L : base + CPR - 1;
-          const int soff = ok ? (rpix + (L - base) * kstep) * rowb : zoff;
+          // empty slots address past the buffer end: the load returns
+          // zeros without a memory request
+          const int soff = ok ? (rpix + (L - base) * kstep) * rowb : zoff + rowb;
           const float4 x = bload4(vr, qoff, soff);
           const float cl = rlf(wl, Lc), cr = rlf(wr, Lc);
           if (gg >= 0) acc[gg] = fma4(cl, x, acc[gg]);
@@ -596,8 +598,8 @@ static bool try_slice(Plan& p, const std::vector<float>& m, int sax, int dax) {
   p.S = S;
   p.SQ = (S + 3) / 4;
   p.U = e == 0 ? p.D0 : p.D1;
-  // The forward reads the canonical volume with one extra zero row, so the
-  // volume is always staged; buffer offsets are 32-bit.
+  // The forward reads the canonical volume staged with one extra zero row
+  // (the buffer it binds ends there); buffer offsets are 32-bit.
   p.vol_direct = false;
   const double rowb = 16.0 * ((S + 3) / 4);
   if ((static_cast<double>(p.P) * p.Q + 1) * rowb >= 2147483647.0) return false;
