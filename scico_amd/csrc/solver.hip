@@ -381,6 +381,14 @@ struct NdIdx {
     n = static_cast<int64_t>(d0) * d1 * d2;
   }
   __device__ void split(int64_t e, int idx[3]) const {
+    if (n <= 0xffffffffll) {  // 32-bit arithmetic below 2^32 elements
+      const unsigned u = static_cast<unsigned>(e), d2 = static_cast<unsigned>(dims[2]);
+      const unsigned q = u / d2;
+      idx[2] = static_cast<int>(u - q * d2);
+      idx[1] = static_cast<int>(q % static_cast<unsigned>(dims[1]));
+      idx[0] = static_cast<int>(q / static_cast<unsigned>(dims[1]));
+      return;
+    }
     idx[0] = static_cast<int>(e / st[0]);
     idx[1] = static_cast<int>((e / dims[2]) % dims[1]);
     idx[2] = static_cast<int>(e % dims[2]);
