@@ -226,22 +226,41 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
         }
         first[gg + 1] = base + cnt;
       }
+      // Groups in chunks of GC: all slot loads of a chunk are issued first
+      // (empty slots address past the buffer end and return zeros without
+      // a memory request), then only the occupied slots are accumulated
+      // (the emptiness test is wave-uniform).
+      constexpr int NG = FB + 1, GC = 5;
 #pragma unroll
-      for (int gg = -1; gg < FB; ++gg) {
+      for (int g0 = 0; g0 < NG; g0 += GC) {
+        float4 xs[GC][SPG];
 #pragma unroll
-        for (int k = 0; k < SPG; ++k) {
-          // Empty slots use the weights of lane base + CPR - 1, which never
-          // holds a candidate (zero weights).
-          const int L = first[gg + 1] + k;
-          const bool ok = L < first[gg + 2];
-          const int Lc = ok ? L : base + CPR - 1;
-          // empty slots address past the buffer end: the load returns
-          // zeros without a memory request
-          const int soff = ok ? (rpix + (L - base) * kstep) * rowb : zoff + rowb;
-          const float4 x = bload4(vr, qoff, soff);
-          const float cl = rlf(wl, Lc), cr = rlf(wr, Lc);
-          if (gg >= 0) acc[gg] = fma4(cl, x, acc[gg]);
-          if (gg + 1 < FB) acc[gg + 1] = fma4(cr, x, acc[gg + 1]);
+        for (int c = 0; c < GC; ++c) {
+          const int gg = g0 + c - 1;
+          if (gg < FB) {
+#pragma unroll
+            for (int k = 0; k < SPG; ++k) {
+              const int L = first[gg + 1] + k;
+              const bool ok = L < first[gg + 2];
+              const int soff = ok ? (rpix + (L - base) * kstep) * rowb : zoff + rowb;
+              xs[c][k] = bload4(vr, qoff, soff);
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < GC; ++c) {
+          const int gg = g0 + c - 1;
+          if (gg < FB) {
+#pragma unroll
+            for (int k = 0; k < SPG; ++k) {
+              const int L = first[gg + 1] + k;
+              if (L < first[gg + 2]) {
+                const float cl = rlf(wl, L), cr = rlf(wr, L);
+                if (gg >= 0) acc[gg] = fma4(cl, xs[c][k], acc[gg]);
+                if (gg + 1 < FB) acc[gg + 1] = fma4(cr, xs[c][k], acc[gg + 1]);
+              }
+            }
+          }
         }
       }
     }
