@@ -384,6 +384,116 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
   }
 }
 
+// The same walk for a lane stride S known at compile time: a block
+// (line group, c0) is the S phases of every chain; lane l's S pixels of a
+// block are contiguous (c0 + S l + r, r < S), loaded together one block
+// ahead into one of two register buffers that alternate between blocks,
+// so the per-phase loop bookkeeping and register copies of the runtime-S
+// walk disappear.
+template <int V, int CH, bool COLFAST, int S>
+__device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float* acc, int L,
+                                           int cnt, const float* Pxmin, const float* P0,
+                                           const float* P1, const double* rw, int M, int N,
+                                           int D) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const unsigned Lf = COLFAST ? M : N;
+  const int Ls = COLFAST ? N : M;
+  if (wave >= Ls) return;
+  const float Dmax = static_cast<float>(D);
+  constexpr unsigned span = 64u * S;
+  const unsigned loff = static_cast<unsigned>(S * lane);
+  int row[CH][V];
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+#pragma unroll
+    for (int c = 0; c < CH; ++c) row[c][v] = ((v * 4 + wave) * CH + c) * L + 2;
+  __amdgpu_buffer_rsrc_t rs[CH];
+  auto set_lines = [&](int s0) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int sc = s0 + 4 * c;
+      const bool has = sc < Ls;
+      rs[c] = make_rsrc(src + static_cast<size_t>(has ? sc : s0) * Lf,
+                        has ? static_cast<int>(Lf * 4u) : 0);
+    }
+  };
+  float t[CH][V];
+  auto set_t = [&](int s0) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const float fs = static_cast<float>(s0 + 4 * c);
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        t[c][v] = COLFAST ? __fmul_rn(P1[v], fs) : __fadd_rn(Pxmin[v], __fmul_rn(P0[v], fs));
+    }
+  };
+  // Inside a line the compiler merges a lane's S dword loads into one or
+  // two wide loads.  A block that runs past the line end keeps one
+  // bounds-checked dword load per pixel, so that exactly the pixels past
+  // the end (and every pixel of a line past the image) read zero, whatever
+  // the range check of a partially out-of-range wide load returns: opaque
+  // offsets prevent the merge, and the work-group coherence scope (still
+  // served by the CU's vector cache) keeps the two load sequences from
+  // being combined into one.
+  auto load = [&](float (&b)[CH][S], unsigned c0) {
+    const unsigned vo = (c0 + loff) * 4u;
+    if (c0 + span <= Lf) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int r = 0; r < S; ++r) b[c][r] = bload1(rs[c], vo + 4u * r, 0);
+    } else {
+      unsigned vb = vo, o[S];
+      asm volatile("" : "+v"(vb));
+#pragma unroll
+      for (int r = 0; r < S; ++r) {
+        o[r] = vb + 4u * r;
+        asm volatile("" : "+v"(o[r]));
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+          b[c][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs[c], o[r], 0, 1));
+    }
+  };
+  int s0 = wave;
+  unsigned c0 = 0;
+  set_lines(s0);
+  set_t(s0);
+  // Accumulate block (s0, c0) from cur after issuing the loads of the next
+  // block into nxt; false once the last block is done.
+  auto step = [&](const float (&cur)[CH][S], float (&nxt)[CH][S]) {
+    int sn = s0;
+    unsigned cn = c0 + span;
+    if (cn >= Lf) {
+      cn = 0;
+      sn += 4 * CH;
+    }
+    const bool more = sn < Ls;
+    if (more) {
+      if (sn != s0) set_lines(sn);
+      load(nxt, cn);
+    }
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+      float val[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) val[c] = cur[c][r];
+      fwd_phase<V, CH, COLFAST>(acc, row, t, val, static_cast<float>(c0 + loff + r), cnt, Pxmin,
+                                P0, P1, rw, Dmax);
+    }
+    if (more && sn != s0) set_t(sn);
+    s0 = sn;
+    c0 = cn;
+    return more;
+  };
+  float ba[CH][S], bb[CH][S];
+  load(ba, 0);
+  while (step(ba, bb) && step(bb, ba)) {
+  }
+}
+
 // One forward group (gi indexes groups[]); acc is the dynamic LDS array.
 template <int V, int CH>
 __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
@@ -415,10 +525,32 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   }
   for (int e = threadIdx.x; e < V * 4 * CH * L; e += 256) acc[e] = 0.0f;
   __syncthreads();
-  if (grp.z & 1)
-    fwd_body<V, CH, true>(imT, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
-  else
-    fwd_body<V, CH, false>(im, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+  // lane strides 3 to 5 cover unit-spaced detectors with pixel sizes
+  // sqrt(2)/2 to 1; other strides take the runtime-stride walk
+  const bool colf = grp.z & 1;
+  const float* src = colf ? imT : im;
+#define SCXR_FWD_S(SS)                                                                 \
+  if (colf)                                                                            \
+    fwd_body_s<V, CH, true, SS>(src, acc, L, cnt, Pxmin, P0, P1, rw, M, N, D);         \
+  else                                                                                 \
+    fwd_body_s<V, CH, false, SS>(src, acc, L, cnt, Pxmin, P0, P1, rw, M, N, D);
+  switch (grp.w) {
+    case 3:
+      SCXR_FWD_S(3)
+      break;
+    case 4:
+      SCXR_FWD_S(4)
+      break;
+    case 5:
+      SCXR_FWD_S(5)
+      break;
+    default:
+      if (colf)
+        fwd_body<V, CH, true>(src, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+      else
+        fwd_body<V, CH, false>(src, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+  }
+#undef SCXR_FWD_S
   __syncthreads();
   double res = 0.0;
 #pragma unroll
