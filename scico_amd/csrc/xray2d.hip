@@ -613,7 +613,8 @@ __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
 // margins, so every pixel's bins ind and ind + 1 lie inside the window.
 struct __align__(16) ViewLds {
   float Pxmin, P0, P1;
-  int base4;  // byte offset in win[] of detector bin 0 for this view
+  float base4;  // byte offset in win_all of detector bin 0 for this view (an
+                // integer below 2^24, so 4 fl + base4 is exact in float32)
   double rw;
   double pad;
 };
@@ -653,7 +654,8 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   const float fi = static_cast<float>(i);
   const float fi0 = static_cast<float>(i0), fj0 = static_cast<float>(j0);
   const int wave = t >> 6, lane = t & 63;
-  const char* winb = reinterpret_cast<const char*>(win);
+  const char* winb = reinterpret_cast<const char*>(&win_all[0][0]);
+  const int gbase4 = grp * (ADJ_VC * ADJ_WFAST * 4);
   // this group's views [vb, ve); every group runs the same number of chunks
   const int share = (A + ADJ_VS - 1) / ADJ_VS;
   const int vb = min(A, grp * share), ve = min(A, vb + share);
@@ -702,11 +704,11 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
             tv.P0 = g.y;
             tv.P1 = g.z;
             tv.rw = rwt[a0 + v];
-            tv.base4 = 4 * (v * W - lop[q]);
+            tv.base4 = static_cast<float>(gbase4 + 4 * (v * W - lop[q]));
           } else {
             tv.Pxmin = tv.P0 = tv.P1 = 0.0f;
             tv.rw = 1.0;
-            tv.base4 = 4 * v * W;
+            tv.base4 = static_cast<float>(gbase4 + 4 * v * W);
           }
           tv.pad = 0.0;
           gv[v] = tv;
@@ -729,7 +731,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
             const float fl = floorf(Px);
             const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
             w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
-            k[u][q] = tv.base4 + (static_cast<int>(fl) << 2);
+            k[u][q] = static_cast<int>(static_cast<unsigned>(__fmaf_rn(fl, 4.0f, tv.base4)));
           }
         }
         float y0[4][PPT], y1[4][PPT];
