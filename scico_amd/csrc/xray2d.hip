@@ -634,7 +634,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   __shared__ float win_all[ADJ_VS][ADJ_VC * ADJ_WFAST];
   __shared__ ViewLds gv_all[ADJ_VS][ADJ_VC];
   __shared__ double red[4 * ADJ_VS];
-  const int grp = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
   float* win = win_all[grp];
   ViewLds* gv = gv_all[grp];
   const int tx = t % ADJ_TJ, ty = t / ADJ_TJ;
@@ -661,23 +661,32 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   const int vb = min(A, grp * share), ve = min(A, vb + share);
   const int nchunk = (share + ADJ_VC - 1) / ADJ_VC;
 
+  // Staging is spread over the lanes: lane 8 sq + sw of a wave handles
+  // view slot sv = wave + 4 sq of a chunk; it computes that view's window
+  // origin and writes word sw of its LDS record.  The windows themselves
+  // are read by the whole wave, one view at a time, through a buffer
+  // resource bounded to the view's sinogram row, so bins outside [0, D)
+  // read zero.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  const int sq = lane >> 3, sw = lane & 7, sv = wv + 4 * sq;
   float pf[VPW];
-  int lop[VPW];
+  int lo_nx = 0;  // window origin of this lane's view slot in the prefetched chunk
   auto prefetch = [&](int a0) {
+    const int a = a0 + sv;
+    const int as = a < ve ? a : 0;
+    const float4 g = geom[as];
+    const int lo = static_cast<int>(floorf(fmaf(g.y, fi0, fmaf(g.z, fj0, clo[as]))));
+    lo_nx = a < ve ? lo : 0;
 #pragma unroll
     for (int q = 0; q < VPW; ++q) {
-      const int a = a0 + wave + 4 * q;
+      const int aq = a0 + wv + 4 * q;
       float val = 0.0f;
-      int lo = 0;
-      if (a < ve) {
-        const float4 g = geom[a];
-        lo = static_cast<int>(floorf(fmaf(g.y, fi0, fmaf(g.z, fj0, clo[a]))));
-        const int b = lo + lane;
-        if (lane < W && static_cast<unsigned>(b) < static_cast<unsigned>(D))
-          val = sino[a * D + b];
+      if (aq < ve) {
+        const int loq = __builtin_amdgcn_readlane(lo, 8 * q);
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(sino + static_cast<size_t>(aq) * D, D * 4);
+        val = bload1(rs, static_cast<unsigned>(loq + lane) * 4u, 0);
       }
       pf[q] = val;
-      lop[q] = lo;
     }
   };
 
@@ -692,28 +701,24 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
     __syncthreads();  // the previous chunk's gathers are done
 #pragma unroll
     for (int q = 0; q < VPW; ++q) {
-      const int v = wave + 4 * q;
-      if (v < nv4) {
-        const bool real = v < nv;
-        if (lane < W) win[v * W + lane] = real ? pf[q] : 0.0f;
-        if (lane == 0) {
-          ViewLds tv;
-          if (real) {
-            const float4 g = geom[a0 + v];
-            tv.Pxmin = g.x;
-            tv.P0 = g.y;
-            tv.P1 = g.z;
-            tv.rw = rwt[a0 + v];
-            tv.base4 = static_cast<float>(gbase4 + 4 * (v * W - lop[q]));
-          } else {
-            tv.Pxmin = tv.P0 = tv.P1 = 0.0f;
-            tv.rw = 1.0;
-            tv.base4 = static_cast<float>(gbase4 + 4 * v * W);
-          }
-          tv.pad = 0.0;
-          gv[v] = tv;
-        }
+      const int v = wv + 4 * q;
+      if (v < nv4 && lane < W) win[v * W + lane] = v < nv ? pf[q] : 0.0f;
+    }
+    if (sv < nv4) {
+      // record words: Pxmin, P0, P1, base4, rw (2 words), pad (2 words);
+      // padded views get zero geometry and rw = 1
+      const bool real = sv < nv;
+      const int a = real ? a0 + sv : 0;
+      uint32_t word = 0;
+      if (sw < 3) {
+        word = real ? reinterpret_cast<const uint32_t*>(geom)[4 * a + sw] : 0u;
+      } else if (sw == 3) {
+        word = __float_as_uint(static_cast<float>(gbase4 + 4 * (sv * W - (real ? lo_nx : 0))));
+      } else if (sw < 6) {
+        word = real ? reinterpret_cast<const uint32_t*>(rwt)[2 * a + (sw - 4)]
+                    : (sw == 5 ? 0x3ff00000u : 0u);
       }
+      reinterpret_cast<uint32_t*>(gv)[8 * sv + sw] = word;
     }
     __syncthreads();
     if (a0 + ADJ_VC < ve) prefetch(a0 + ADJ_VC);
