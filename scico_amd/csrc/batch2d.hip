@@ -135,6 +135,8 @@ int create(Batch& b, int ns, const int32_t* dims, const float* angles, const dou
       if (rc) return rc;
     }
     b.bt.ppt = ppt;
+    b.bt.fract = 1;
+    for (int s = 0; s < ns; ++s) b.bt.fract = b.bt.fract && tabs[s].px_nonneg;
   }
   {
     int Dm = 0;

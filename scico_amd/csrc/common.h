@@ -62,6 +62,13 @@ __device__ __forceinline__ float div_rn(float d, double rw) {
   return static_cast<float>(static_cast<double>(d) * rw);
 }
 
+// floor(x) converted to int32 in one instruction (for |x| < 2^31).
+__device__ __forceinline__ int cvt_flr_i32(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // Sum of one double per thread over a 256-thread workgroup (fixed order:
 // wave shuffles, then the 4 wave sums left to right).  Thread 0 gets the
 // result.  `sh` holds 4 doubles of LDS.

@@ -99,6 +99,17 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
     }
   }
 
+  // Px >= 0 for every pixel of every view (bounds of Px over the image in
+  // float64, with a margin far above the float32 rounding of Px): then
+  // Px - floor(Px) is the exact fraction and the adjoint may take it from
+  // the fraction instruction.
+  h.px_nonneg = true;
+  for (int a = 0; a < A; ++a) {
+    const double lo = static_cast<double>(geom[a].x) + std::min<double>(geom[a].y, 0.0) * (M - 1) +
+                      std::min<double>(geom[a].z, 0.0) * (N - 1);
+    h.px_nonneg = h.px_nonneg && lo >= 0.01;
+  }
+
   // Adjoint: window origin offset per view (see adj_tile): a lower bound
   // of floor(Px) - 1 over any tile; the window spans the projected tile
   // extent plus the lower margin, the rounding of Px and bin ind + 1.  Two
@@ -148,6 +159,7 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg, 0);
   if (rc) return rc;
   p.adj_ppt = h.adj_ppt;
+  p.adj_fract = h.px_nonneg;
   p.host_geom = h.geom;
   p.any_colfast = h.any_colfast;
   p.n_groups = static_cast<int>(h.groups.size());
@@ -613,8 +625,10 @@ __global__ __launch_bounds__(256) void fwd_kernel(const float* __restrict__ im,
 // margins, so every pixel's bins ind and ind + 1 lie inside the window.
 struct __align__(16) ViewLds {
   float Pxmin, P0, P1;
-  float base4;  // byte offset in win_all of detector bin 0 for this view (an
-                // integer below 2^24, so 4 fl + base4 is exact in float32)
+  // byte offset in win_all of detector bin 0 for this view: an integer
+  // (float32 bits when the weights use floor, so that 4 fl + base4 is one
+  // exact float multiply-add; int32 when they use the fraction instruction)
+  uint32_t base4;
   double rw;
   double pad;
 };
@@ -623,7 +637,7 @@ struct __align__(16) ViewLds {
 // dot_i).  The workgroup is ADJ_VS groups of 256 threads over the same
 // pixels; group g takes the g-th contiguous share of the views, and the
 // groups' sums are added in group order at the end.
-template <int PPT>
+template <int PPT, bool FR>
 __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* __restrict__ out,
                                          const float4* __restrict__ geom,
                                          const double* __restrict__ rwt,
@@ -713,7 +727,8 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
       if (sw < 3) {
         word = real ? reinterpret_cast<const uint32_t*>(geom)[4 * a + sw] : 0u;
       } else if (sw == 3) {
-        word = __float_as_uint(static_cast<float>(gbase4 + 4 * (sv * W - (real ? lo_nx : 0))));
+        const int b4 = gbase4 + 4 * (sv * W - (real ? lo_nx : 0));
+        word = FR ? static_cast<uint32_t>(b4) : __float_as_uint(static_cast<float>(b4));
       } else if (sw < 6) {
         word = real ? reinterpret_cast<const uint32_t*>(rwt)[2 * a + (sw - 4)]
                     : (sw == 5 ? 0x3ff00000u : 0u);
@@ -733,10 +748,18 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
 #pragma unroll
           for (int q = 0; q < PPT; ++q) {
             const float Px = __fadd_rn(ti, __fmul_rn(tv.P1, fj[q]));
-            const float fl = floorf(Px);
-            const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
-            w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
-            k[u][q] = static_cast<int>(static_cast<unsigned>(__fmaf_rn(fl, 4.0f, tv.base4)));
+            if (FR) {
+              // Px >= 0: fract(Px) = Px - floor(Px) exactly
+              const float dd = __fsub_rn(1.0f, __builtin_amdgcn_fractf(Px));
+              w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
+              k[u][q] = (cvt_flr_i32(Px) << 2) + static_cast<int>(tv.base4);
+            } else {
+              const float fl = floorf(Px);
+              const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
+              w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
+              k[u][q] = static_cast<int>(
+                  static_cast<unsigned>(__fmaf_rn(fl, 4.0f, __uint_as_float(tv.base4))));
+            }
           }
         }
         float y0[4][PPT], y1[4][PPT];
@@ -803,7 +826,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   }
 }
 
-template <int PPT>
+template <int PPT, bool FR>
 __global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4))) void adj_kernel(const float* __restrict__ sino,
                                                   float* __restrict__ out,
                                                   const float4* __restrict__ geom,
@@ -812,7 +835,7 @@ __global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4)
                                                   int A, int D, int W, AdjEpi epi,
                                                   const int32_t* pred) {
   if (pred_off(pred)) return;
-  adj_tile<PPT>(sino, out, geom, rwt, clo, M, N, A, D, W, epi, blockIdx.x, blockIdx.y,
+  adj_tile<PPT, FR>(sino, out, geom, rwt, clo, M, N, A, D, W, epi, blockIdx.x, blockIdx.y,
                 blockIdx.y * gridDim.x + blockIdx.x);
 }
 
@@ -907,7 +930,7 @@ __global__ __launch_bounds__(256) void fwd_batch_kernel(BatchTabs t, const float
   }
 }
 
-template <int PPT>
+template <int PPT, bool FR>
 __global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4))) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
                                                         float* __restrict__ out, BatchEpi be,
                                                         const int32_t* pred, int pstride) {
@@ -926,7 +949,7 @@ __global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4)
     e.z = be.z ? be.z + 2 * io : nullptr;
     e.u = be.u ? be.u + 2 * io : nullptr;
     e.dot_ws = be.dot_ws ? be.dot_ws + t.toff[s] : nullptr;
-    adj_tile<PPT>(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y, sh.z,
+    adj_tile<PPT, FR>(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y, sh.z,
                   sh.w, me.x, e, w.y % me.z, w.y / me.z, w.y);
     __syncthreads();
   }
@@ -969,12 +992,21 @@ int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im
 int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
                   const BatchEpi& epi, const int32_t* pred, int pstride, hipStream_t s) {
   const unsigned g = grid > 0 ? std::min(grid, t.na) : t.na;
-  if (t.ppt == 2)
-    hipLaunchKernelGGL(adj_batch_kernel<2>, dim3(g), dim3(256 * ADJ_VS), 0, s, t, sino, im, epi, pred,
-                       pstride);
-  else
-    hipLaunchKernelGGL(adj_batch_kernel<1>, dim3(g), dim3(256 * ADJ_VS), 0, s, t, sino, im, epi, pred,
-                       pstride);
+#define SCXR_ADJB(PP, FF)                                                                   \
+  hipLaunchKernelGGL((adj_batch_kernel<PP, FF>), dim3(g), dim3(256 * ADJ_VS), 0, s, t, sino, im, \
+                     epi, pred, pstride)
+  if (t.ppt == 2) {
+    if (t.fract)
+      SCXR_ADJB(2, true);
+    else
+      SCXR_ADJB(2, false);
+  } else {
+    if (t.fract)
+      SCXR_ADJB(1, true);
+    else
+      SCXR_ADJB(1, false);
+  }
+#undef SCXR_ADJB
   SCXR_LAUNCHED("xr2d batch adjoint");
   return SCXR_OK;
 }
@@ -1013,12 +1045,21 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
 int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,
             const AdjEpi& epi) {
   dim3 g(ceil_div(p.N, ADJ_TJ * p.adj_ppt), ceil_div(p.M, ADJ_TI));
-  if (p.adj_ppt == 2)
-    hipLaunchKernelGGL(adj_kernel<2>, g, dim3(256 * ADJ_VS), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
-                       p.N, p.A, p.D, p.adj_w, epi, pred);
-  else
-    hipLaunchKernelGGL(adj_kernel<1>, g, dim3(256 * ADJ_VS), 0, s, sino, im, p.geom, p.rw, p.clo, p.M,
-                       p.N, p.A, p.D, p.adj_w, epi, pred);
+#define SCXR_ADJ(PP, FF)                                                                    \
+  hipLaunchKernelGGL((adj_kernel<PP, FF>), g, dim3(256 * ADJ_VS), 0, s, sino, im, p.geom, p.rw,  \
+                     p.clo, p.M, p.N, p.A, p.D, p.adj_w, epi, pred)
+  if (p.adj_ppt == 2) {
+    if (p.adj_fract)
+      SCXR_ADJ(2, true);
+    else
+      SCXR_ADJ(2, false);
+  } else {
+    if (p.adj_fract)
+      SCXR_ADJ(1, true);
+    else
+      SCXR_ADJ(1, false);
+  }
+#undef SCXR_ADJ
   SCXR_LAUNCHED("xr2d adjoint");
   return SCXR_OK;
 }
