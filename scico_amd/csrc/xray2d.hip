@@ -94,6 +94,18 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
       SCXR_REQUIRE(S <= FWD_SMAX, SCXR_ERR_GEOM,
                    "projected pixel spacing too small for the forward projector "
                    "(lane stride %d > %d)", S, FWD_SMAX);
+      // A larger stride keeps the footprints disjoint too; among the
+      // compiled strides take the one with the fewest phases per line,
+      // ceil(Lf / 64 S) * S (e.g. 4 rather than 3 for 512-pixel lines: 8
+      // phases instead of 9).
+      if (S <= 5) {
+        const int Lf = cls ? M : N;
+        auto phases = [&](int s2) { return (Lf + 64 * s2 - 1) / (64 * s2) * s2; };
+        int best = S;
+        for (int s2 = S + 1; s2 <= 5; ++s2)
+          if (phases(s2) < phases(best)) best = s2;
+        S = best;
+      }
       h.groups.push_back(make_int4(static_cast<int>(h.order.size()), cnt, cls, S));
       for (int k = 0; k < cnt; ++k) h.order.push_back(list[s + k]);
     }
