@@ -238,6 +238,13 @@ __device__ __forceinline__ Weight weight_at(float Pxmin, float P0, float P1, flo
   return Weight{fl, fminf(div_rn(dd, rw), 1.0f)};
 }
 
+// The same weight from the fraction instruction, for Px >= 0, where
+// fract(Px) = Px - floor(Px) exactly (the adjoint's path when every
+// projected coordinate of the plan is non-negative).
+__device__ __forceinline__ float weight_frac(float Px, double rw) {
+  return fminf(div_rn(__fsub_rn(1.0f, __builtin_amdgcn_fractf(Px)), rw), 1.0f);
+}
+
 // Transpose (M, N) -> (N, M) through a padded LDS tile (tile row by,
 // column bx).
 __device__ __forceinline__ void transpose_tile(const float* __restrict__ in,
@@ -761,9 +768,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
           for (int q = 0; q < PPT; ++q) {
             const float Px = __fadd_rn(ti, __fmul_rn(tv.P1, fj[q]));
             if (FR) {
-              // Px >= 0: fract(Px) = Px - floor(Px) exactly
-              const float dd = __fsub_rn(1.0f, __builtin_amdgcn_fractf(Px));
-              w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
+              w[u][q] = weight_frac(Px, tv.rw);
               k[u][q] = (cvt_flr_i32(Px) << 2) + static_cast<int>(tv.base4);
             } else {
               const float fl = floorf(Px);
@@ -882,7 +887,7 @@ __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds
                                                       float* __restrict__ wts,
                                                       const float4* __restrict__ geom,
                                                       const double* __restrict__ rwt, int M,
-                                                      int N, int A) {
+                                                      int N, int A, int fract) {
   const size_t n = static_cast<size_t>(M) * N;
   const size_t e = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
   if (e >= n * A) return;
@@ -890,6 +895,13 @@ __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds
   const size_t p = e - static_cast<size_t>(a) * n;
   const int i = static_cast<int>(p / N), j = static_cast<int>(p - static_cast<size_t>(i) * N);
   const float4 g = geom[a];
+  if (fract) {  // the adjoint's arithmetic for plans with Px >= 0 everywhere
+    const float Px = __fadd_rn(__fadd_rn(g.x, __fmul_rn(g.y, static_cast<float>(i))),
+                               __fmul_rn(g.z, static_cast<float>(j)));
+    inds[e] = cvt_flr_i32(Px);
+    wts[e] = weight_frac(Px, rwt[a]);
+    return;
+  }
   const Weight wt =
       weight_at(g.x, g.y, g.z, g.w, rwt[a], static_cast<float>(i), static_cast<float>(j));
   inds[e] = static_cast<int32_t>(wt.fl);
@@ -1188,7 +1200,7 @@ int scxr_xr2d_weights(const scxr_xr2d_plan* plan, int32_t* inds, float* weights,
   SCXR_REQUIRE(total < (static_cast<size_t>(1) << 40), SCXR_ERR_SHAPE, "too many entries");
   hipLaunchKernelGGL(xr2d::weights_kernel, dim3(ceil_div(static_cast<long long>(total), 256)),
                      dim3(256), 0, as_stream(stream), inds, weights, p.geom, p.rw, p.M, p.N,
-                     p.A);
+                     p.A, static_cast<int>(p.adj_fract));
   SCXR_LAUNCHED("xr2d weights");
   return SCXR_OK;
 }
