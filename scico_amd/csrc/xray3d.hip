@@ -53,7 +53,7 @@ struct Plan {
   bool vol_direct = false, proj_direct = false;
   float* mats = nullptr;   // (V, 2, 4) float32
   float4* geom = nullptr;  // slice mode: (a, b, t, 0) in-plane row per view
-  float* volc = nullptr;   // canonical volume workspace (P Q + 1 rows of 4 SQ; last row 0)
+  float* volc = nullptr;   // canonical volume workspace (P Q rows of 4 SQ floats)
   float* proja = nullptr;  // canonical projection workspace (V U 4 SQ)
   struct GenView* gview = nullptr;  // general mode: per-view gather set-up
 };
@@ -148,8 +148,9 @@ __device__ __forceinline__ float rlf(float v, int lane) {
 // candidate of every bin group g (ul = u0 + g, g = -1 .. FB - 1).  Each
 // group has SPG static slots (a group holds at most SPG pixels: pixel
 // centres are at least |f| apart along the fast axis), so the whole row is
-// straight-line code: every slot's row of slices is loaded independently
-// and accumulated into static registers, the empty slots skipped.
+// straight-line code apart from wave-uniform branches: every occupied
+// slot's row of slices is loaded independently and accumulated into
+// static registers.
 template <int CPR, int SPG>
 __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict__ vol,
                                                         float4* __restrict__ proj,
@@ -166,11 +167,9 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
   const unsigned quad = blockIdx.z * 64 + lane;
   const bool qok = quad < static_cast<unsigned>(SQ);
   const unsigned qoff = (qok ? quad : 0u) * 16u;
-  // vol holds P Q rows of SQ slice quads plus one zero row; the buffer
-  // bound ends after that row
+  // vol holds P Q rows of SQ slice quads
   const int rowb = SQ * 16;
-  const int zoff = P * Q * rowb;
-  const __amdgpu_buffer_rsrc_t vr = make_rsrc(vol, zoff + rowb);
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(vol, P * Q * rowb);
   const float4 g = geom[v];
   const float a = g.x, b = g.y, t = g.z;
   const bool qfast = fabsf(b) >= fabsf(a);
@@ -226,10 +225,11 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
         }
         first[gg + 1] = base + cnt;
       }
-      // Groups in chunks of GC: all slot loads of a chunk are issued first
-      // (empty slots address past the buffer end and return zeros without
-      // a memory request), then only the occupied slots are accumulated
-      // (the emptiness test is wave-uniform).
+      // Groups in chunks of GC: the loads of a chunk's occupied slots are
+      // issued first, then those slots are accumulated.  Emptiness is
+      // wave-uniform, so empty slots are skipped by scalar branches: they
+      // issue no load (an out-of-range load would still take its cycles
+      // in the texture addresser, which bounds this kernel).
       constexpr int NG = FB + 1, GC = 5;
 #pragma unroll
       for (int g0 = 0; g0 < NG; g0 += GC) {
@@ -241,9 +241,10 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
 #pragma unroll
             for (int k = 0; k < SPG; ++k) {
               const int L = first[gg + 1] + k;
-              const bool ok = L < first[gg + 2];
-              const int soff = ok ? (rpix + (L - base) * kstep) * rowb : zoff + rowb;
-              xs[c][k] = bload4(vr, qoff, soff);
+              if (L < first[gg + 2])
+                xs[c][k] = bload4(vr, qoff, (rpix + (L - base) * kstep) * rowb);
+              else
+                xs[c][k] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
           }
         }
@@ -617,11 +618,11 @@ static bool try_slice(Plan& p, const std::vector<float>& m, int sax, int dax) {
   p.S = S;
   p.SQ = (S + 3) / 4;
   p.U = e == 0 ? p.D0 : p.D1;
-  // The forward reads the canonical volume staged with one extra zero row
-  // (the buffer it binds ends there); buffer offsets are 32-bit.
+  // The forward reads the canonical (slice-fastest) volume through one
+  // buffer resource; buffer offsets are 32-bit.
   p.vol_direct = false;
   const double rowb = 16.0 * ((S + 3) / 4);
-  if ((static_cast<double>(p.P) * p.Q + 1) * rowb >= 2147483647.0) return false;
+  if (static_cast<double>(p.P) * p.Q * rowb >= 2147483647.0) return false;
   const int Ue = e == 0 ? p.D0 : p.D1;
   if (static_cast<double>(p.V) * Ue * rowb >= 2147483647.0) return false;
   p.proj_direct = dax == 1 && delta == 0 && p.D1 == S && S % 4 == 0;
@@ -716,9 +717,8 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
     const size_t s4 = 4 * static_cast<size_t>(p.SQ);
     {
       const size_t rows = static_cast<size_t>(p.P) * p.Q;
-      rc = dev_alloc(reinterpret_cast<void**>(&p.volc), (rows + 1) * s4 * sizeof(float));
+      rc = dev_alloc(reinterpret_cast<void**>(&p.volc), rows * s4 * sizeof(float));
       if (rc) return rc;
-      SCXR_HIP(hipMemset(p.volc + rows * s4, 0, s4 * sizeof(float)));
     }
     if (!p.proj_direct) {
       rc = dev_alloc(reinterpret_cast<void**>(&p.proja),
