@@ -57,6 +57,46 @@ void view_geometry(int A, const float* angles, const double* x0, const double* d
   }
 }
 
+// LDS bank model of one forward phase: lane l updates bins b and b + 1 with
+// b = floor(p + d pos(l)), pos(l) = (l k) mod 64.  An LDS access serves 32
+// lanes at a time over 32 banks, one word per bank per cycle, so each
+// half-wave and word costs the largest number of its lanes that share a
+// bank; the sum over both halves and both words is returned, averaged over
+// fixed offsets p.  The cost depends only on |d| and k; it is tabulated
+// for |d| in steps of 1/256.
+static int lane_mult_for(double d) {
+  thread_local std::vector<signed char> memo;
+  d = std::fabs(d);
+  const int key = static_cast<int>(std::lround(d * 256.0));
+  if (key < 0 || key >= 64 * 256) return 1;
+  if (memo.empty()) memo.assign(64 * 256, 0);
+  if (memo[key]) return memo[key];
+  const double dq = key / 256.0;
+  int best_k = 1;
+  long best = -1;
+  for (int k = 1; k < 64; k += 2) {
+    long cost = 0;
+    for (int j = 0; j < 12; ++j) {
+      const double p0 = 0.37 + 5.31 * j;
+      for (int word = 0; word < 2; ++word)
+        for (int half = 0; half < 2; ++half) {
+          int cnt[32] = {0}, mx = 0;
+          for (int l = 32 * half; l < 32 * half + 32; ++l) {
+            const long b = static_cast<long>(std::floor(p0 + dq * ((l * k) & 63))) + word;
+            mx = std::max(mx, ++cnt[b & 31]);
+          }
+          cost += mx;
+        }
+    }
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_k = k;
+    }
+  }
+  memo[key] = static_cast<signed char>(best_k);
+  return best_k;
+}
+
 int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
                  double y0, const float* angles, int vpg, int ppt) {
   view_geometry(A, angles, x0, dx, y0, h.geom, h.rw);
@@ -106,7 +146,12 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
           if (phases(s2) < phases(best)) best = s2;
         S = best;
       }
-      h.groups.push_back(make_int4(static_cast<int>(h.order.size()), cnt, cls, S));
+      // Lane permutation (position (lane k) mod 64 for an odd k) that the
+      // bank model prefers for the group's first view: the phase's set of
+      // pixels, hence the disjointness of the footprints, is unchanged.
+      const float4 g0 = geom[list[s]];
+      const int kmul = lane_mult_for(S * static_cast<double>(cls ? g0.y : g0.z));
+      h.groups.push_back(make_int4(static_cast<int>(h.order.size()), cnt, cls | (kmul << 1), S));
       for (int k = 0; k < cnt; ++k) h.order.push_back(list[s + k]);
     }
   }
@@ -338,16 +383,16 @@ __device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
 // accumulated.
 template <int V, int CH, bool COLFAST>
 __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* acc, int L,
-                                         int cnt, int S, const float* Pxmin, const float* P0,
-                                         const float* P1, const double* rw, int M, int N,
-                                         int D) {
+                                         int cnt, int S, int kmul, const float* Pxmin,
+                                         const float* P0, const float* P1, const double* rw,
+                                         int M, int N, int D) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const unsigned Lf = COLFAST ? M : N;  // fast (lane) axis length
   const int Ls = COLFAST ? N : M;
   if (wave >= Ls) return;
   const float Dmax = static_cast<float>(D);
   const unsigned span = 64u * S;
-  const unsigned loff = static_cast<unsigned>(S * lane);
+  const unsigned loff = static_cast<unsigned>(S * ((lane * kmul) & 63));
   int row[CH][V];
 #pragma unroll
   for (int v = 0; v < V; ++v)
@@ -423,7 +468,7 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
 // walk disappear.
 template <int V, int CH, bool COLFAST, int S>
 __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float* acc, int L,
-                                           int cnt, const float* Pxmin, const float* P0,
+                                           int cnt, int kmul, const float* Pxmin, const float* P0,
                                            const float* P1, const double* rw, int M, int N,
                                            int D) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -432,7 +477,7 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
   if (wave >= Ls) return;
   const float Dmax = static_cast<float>(D);
   constexpr unsigned span = 64u * S;
-  const unsigned loff = static_cast<unsigned>(S * lane);
+  const unsigned loff = static_cast<unsigned>(S * ((lane * kmul) & 63));
   int row[CH][V];
 #pragma unroll
   for (int v = 0; v < V; ++v)
@@ -559,12 +604,13 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   // lane strides 3 to 5 cover unit-spaced detectors with pixel sizes
   // sqrt(2)/2 to 1; other strides take the runtime-stride walk
   const bool colf = grp.z & 1;
+  const int kmul = grp.z >> 1;  // lane permutation multiplier (odd)
   const float* src = colf ? imT : im;
 #define SCXR_FWD_S(SS)                                                                 \
   if (colf)                                                                            \
-    fwd_body_s<V, CH, true, SS>(src, acc, L, cnt, Pxmin, P0, P1, rw, M, N, D);         \
+    fwd_body_s<V, CH, true, SS>(src, acc, L, cnt, kmul, Pxmin, P0, P1, rw, M, N, D);   \
   else                                                                                 \
-    fwd_body_s<V, CH, false, SS>(src, acc, L, cnt, Pxmin, P0, P1, rw, M, N, D);
+    fwd_body_s<V, CH, false, SS>(src, acc, L, cnt, kmul, Pxmin, P0, P1, rw, M, N, D);
   switch (grp.w) {
     case 3:
       SCXR_FWD_S(3)
@@ -577,9 +623,9 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
       break;
     default:
       if (colf)
-        fwd_body<V, CH, true>(src, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+        fwd_body<V, CH, true>(src, acc, L, cnt, grp.w, kmul, Pxmin, P0, P1, rw, M, N, D);
       else
-        fwd_body<V, CH, false>(src, acc, L, cnt, grp.w, Pxmin, P0, P1, rw, M, N, D);
+        fwd_body<V, CH, false>(src, acc, L, cnt, grp.w, kmul, Pxmin, P0, P1, rw, M, N, D);
   }
 #undef SCXR_FWD_S
   __syncthreads();
