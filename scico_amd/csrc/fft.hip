@@ -191,8 +191,9 @@ __global__ __launch_bounds__(256) void pass_kernel(const void* __restrict__ in,
   }
 }
 
-// Power-of-two lengths: Stockham auto-sort passes of radix 4 (and one of
-// radix 2 when log2 n is odd) on lpb lines held in one LDS buffer.  Each
+// Power-of-two lengths: Stockham auto-sort passes of radix 16 (after one
+// pass of radix 2, 4 or 8 for log2 n mod 4) on lpb lines held in one LDS
+// buffer.  Each
 // thread takes whole butterflies: it reads its R inputs x[j + r n / R] into
 // registers, twiddles them by w^(k r) (k = j mod Ns, w = exp(-2 pi i /
 // (R Ns)), taken from the n-entry table), transforms them and, after a
@@ -202,9 +203,76 @@ __global__ __launch_bounds__(256) void pass_kernel(const void* __restrict__ in,
 constexpr int SH_THREADS = 256;
 constexpr int SH_MAXV = 32;  // values per thread per stage (lpb n <= 8192)
 
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y));
+}
+
+// 4-point DFT of a[0], a[s], a[2 s], a[3 s] in place (natural order).
+template <int S, int N>
+__device__ __forceinline__ void dft4_strided(float2 (&v)[N], int o, bool inv) {
+  const float2 s02 = cadd(v[o], v[o + 2 * S]), d02 = csub(v[o], v[o + 2 * S]);
+  const float2 s13 = cadd(v[o + S], v[o + 3 * S]), d13 = csub(v[o + S], v[o + 3 * S]);
+  // -i d13 (forward) or +i d13 (inverse)
+  const float2 t = inv ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+  v[o] = cadd(s02, s13);
+  v[o + 2 * S] = csub(s02, s13);
+  v[o + S] = cadd(d02, t);
+  v[o + 3 * S] = csub(d02, t);
+}
+
+// exp(-+2 pi i m / 16) for m = 0 .. 9 (forward sign; conjugated for the
+// inverse), the internal twiddles of the 8- and 16-point butterflies.
+__device__ __forceinline__ float2 w16(int m, bool inv) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f;
+  constexpr float h = 0.70710678118654752f;
+  float2 w;
+  switch (m) {
+    case 0: w = make_float2(1.0f, 0.0f); break;
+    case 1: w = make_float2(c1, -s1); break;
+    case 2: w = make_float2(h, -h); break;
+    case 3: w = make_float2(s1, -c1); break;
+    case 4: w = make_float2(0.0f, -1.0f); break;
+    case 6: w = make_float2(-h, -h); break;
+    default: w = make_float2(-c1, s1); break;  // m = 9
+  }
+  return inv ? make_float2(w.x, -w.y) : w;
+}
+
 template <int R>
 __device__ __forceinline__ void dft_small(float2 (&v)[R], bool inv) {
-  if constexpr (R == 2) {
+  if constexpr (R == 16 || R == 8) {
+    // R = 4 Q: inputs n1 + 4 n2 ... as R = Q x 4 with Q = R / 4:
+    // DFT over the Q-strided groups, twiddle, then 4- (or 2-) point DFTs.
+    constexpr int Q = R / 4;  // 4 (R = 16) or 2 (R = 8)
+    // step 1: for each n1 < Q, 4-point DFT over n2 of v[n1 + Q n2]
+#pragma unroll
+    for (int n1 = 0; n1 < Q; ++n1) dft4_strided<Q, R>(v, n1, inv);
+    // step 2: twiddle v[n1 + Q k2] by W_R^(n1 k2)
+#pragma unroll
+    for (int n1 = 1; n1 < Q; ++n1)
+#pragma unroll
+      for (int k2 = 1; k2 < 4; ++k2) v[n1 + Q * k2] = cmul(v[n1 + Q * k2], w16(n1 * k2 * (16 / R), inv));
+    // step 3: for each k2, Q-point DFT over n1; output k = k2 + 4 k1
+    float2 o[R];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      if constexpr (Q == 4) {
+        float2 g[4] = {v[Q * k2], v[1 + Q * k2], v[2 + Q * k2], v[3 + Q * k2]};
+        dft4_strided<1, 4>(g, 0, inv);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) o[k2 + 4 * k1] = g[k1];
+      } else {
+        const float2 a = v[Q * k2], b = v[1 + Q * k2];
+        o[k2] = cadd(a, b);
+        o[k2 + 4] = csub(a, b);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = o[r];
+  } else if constexpr (R == 2) {
     const float2 a = v[0], b = v[1];
     v[0] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
     v[1] = make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y));
@@ -262,12 +330,15 @@ __device__ __forceinline__ void stockham_stage(float2* X, int n, int lpb, int Ns
 
 __device__ __forceinline__ void stockham(float2* X, int n, int log2n, int lpb,
                                          const float2* tw, bool inv) {
+  // one stage of radix 2, 4 or 8 for log2 n mod 4, then radix-16 stages
   int Ns = 1;
-  if (log2n & 1) {
-    stockham_stage<2>(X, n, lpb, Ns, tw, inv);
-    Ns = 2;
+  switch (log2n & 3) {
+    case 1: stockham_stage<2>(X, n, lpb, Ns, tw, inv); Ns = 2; break;
+    case 2: stockham_stage<4>(X, n, lpb, Ns, tw, inv); Ns = 4; break;
+    case 3: stockham_stage<8>(X, n, lpb, Ns, tw, inv); Ns = 8; break;
+    default: break;
   }
-  for (; Ns < n; Ns *= 4) stockham_stage<4>(X, n, lpb, Ns, tw, inv);
+  for (; Ns < n; Ns *= 16) stockham_stage<16>(X, n, lpb, Ns, tw, inv);
 }
 
 __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __restrict__ in,
@@ -275,25 +346,14 @@ __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __rest
                                                               float scale,
                                                               const int32_t* pred) {
   if (pred_off(pred)) return;
-  extern __shared__ float2 lds[];  // lpb * n values, then the n twiddles
+  extern __shared__ float2 lds[];  // lpb * n values
   __shared__ long long base[64];
   __shared__ long long hoff[64];  // spectrum offset of each line (fused pass)
   const int n = P.n, lpb = P.lpb, tot = lpb * n;
   float2* X = lds;
-  float2* twl = lds + tot;
-  {
-    float2 tv[SH_MAXV / 2];  // n <= 4096: at most 16 per thread, all loads issued first
-#pragma unroll
-    for (int q = 0; q < SH_MAXV / 2; ++q) {
-      const int e = threadIdx.x + q * SH_THREADS;
-      if (e < n) tv[q] = P.tw[e];
-    }
-#pragma unroll
-    for (int q = 0; q < SH_MAXV / 2; ++q) {
-      const int e = threadIdx.x + q * SH_THREADS;
-      if (e < n) twl[e] = tv[q];
-    }
-  }
+  // twiddles are read from the n-entry table in global memory (it stays in
+  // the vector cache; staging it in LDS per workgroup cost more)
+  const float2* twl = P.tw;
   const bool contiguous = P.stride == 1;
   // Strided passes: workgroups dispatched to one XCD (consecutive ids go to
   // the 8 XCDs in turn) take consecutive line groups, so the cache lines
@@ -454,7 +514,7 @@ int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool
     // Stockham: one LDS buffer of lpb lines (at most 8192 values); lines per
     // workgroup chosen so that a pass has at least ~1024 workgroups.
     const long long lines = p.batch * (p.fsize / t.n);
-    int lpb = std::max(1, 8192 / t.n - 1);  // plus one line's worth of twiddles
+    int lpb = std::max(1, 8192 / t.n);
     { int q = 1; while (q * 2 <= lpb) q *= 2; lpb = q; }
     while (lpb > 1 && (lines + lpb - 1) / lpb < 1024) lpb /= 2;
     lpb = std::min(lpb, 64);
@@ -467,7 +527,7 @@ int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool
     P.tw = t.tw;
     P.h = p.h;
     const float scale = (inverse || fused) ? 1.0f / static_cast<float>(t.n) : 1.0f;
-    const size_t lds = sizeof(float2) * (lpb + 1) * t.n;
+    const size_t lds = sizeof(float2) * lpb * t.n;
     const unsigned grid = static_cast<unsigned>((P.lines + lpb - 1) / lpb);
     hipLaunchKernelGGL(stockham_kernel, dim3(grid), dim3(SH_THREADS), lds, s, in, out, P, scale,
                        pred);
