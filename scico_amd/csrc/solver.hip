@@ -687,6 +687,67 @@ __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ 
   }
 }
 
+// Two-dimensional (M, N) case on 16 x 16 output tiles.  The padded image
+// around the tile (padded rows and columns i0 - 1 .. i0 + 16, taken modulo
+// P and edge-replicated) is staged in LDS, the (shrunk) coefficients of the
+// 17 x 17 padded locations the tile needs are computed once each instead
+// of once per neighbouring output, and every output combines the
+// coefficients at p, p - e0 and p - e1 with the same operations, in the
+// same order, as tv_prox_kernel (so the results are bitwise equal).
+constexpr int TVT = 16;
+__global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict__ v,
+                                                        float* __restrict__ out, TvDims t,
+                                                        float tau, float invK, int iso,
+                                                        const float* __restrict__ w,
+                                                        const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float vs[TVT + 2][TVT + 3];
+  __shared__ float cH[2][TVT + 1][TVT + 2], cL[2][TVT + 1][TVT + 2];
+  const float s = 0.70710677f;
+  const int M = t.d[1], N = t.d[2], P0 = t.P[1], P1 = t.P[2];
+  const int i0 = blockIdx.y * TVT, j0 = blockIdx.x * TVT;
+  const int tid = threadIdx.x;
+  auto wrap = [](int q, int P) { q %= P; return q < 0 ? q + P : q; };
+  for (int e = tid; e < (TVT + 2) * (TVT + 2); e += 256) {
+    const int r = e / (TVT + 2), c = e - r * (TVT + 2);
+    const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + c, P1);
+    vs[r][c] = v[static_cast<int64_t>(min(qi, M - 1)) * N + min(qj, N - 1)];
+  }
+  __syncthreads();
+  for (int e = tid; e < (TVT + 1) * (TVT + 1); e += 256) {
+    const int r = e / (TVT + 1), c = e - r * (TVT + 1);
+    const float vq = vs[r][c], vn0 = vs[r + 1][c], vn1 = vs[r][c + 1];
+    float H[2] = {__fmul_rn(s, __fsub_rn(vn0, vq)), __fmul_rn(s, __fsub_rn(vn1, vq))};
+    const float L0 = __fmul_rn(s, __fadd_rn(vq, vn0)), L1 = __fmul_rn(s, __fadd_rn(vq, vn1));
+    const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + c, P1);
+    if (t.circ || (qi < M && qj < N)) {
+      const float tq = w ? __fmul_rn(tau, w[static_cast<int64_t>(qi) * N + qj]) : tau;
+      if (iso) {
+        l21_shrink(H, 2, tq);
+      } else {
+        H[0] = l1_shrink(H[0], tq);
+        H[1] = l1_shrink(H[1], tq);
+      }
+    }
+    cH[0][r][c] = H[0];
+    cH[1][r][c] = H[1];
+    cL[0][r][c] = L0;
+    cL[1][r][c] = L1;
+  }
+  __syncthreads();
+  const int ty = tid / TVT, tx = tid - ty * TVT;
+  const int pi = i0 + ty, pj = j0 + tx;
+  if (pi < M && pj < N) {
+    // p at (ty + 1, tx + 1); p - e0 at (ty, tx + 1); p - e1 at (ty + 1, tx)
+    const float lo0 = __fmul_rn(s, __fadd_rn(cL[0][ty + 1][tx + 1], cL[0][ty][tx + 1]));
+    const float hi0 = __fmul_rn(s, __fsub_rn(cH[0][ty][tx + 1], cH[0][ty + 1][tx + 1]));
+    const float lo1 = __fmul_rn(s, __fadd_rn(cL[1][ty + 1][tx + 1], cL[1][ty + 1][tx]));
+    const float hi1 = __fmul_rn(s, __fsub_rn(cH[1][ty + 1][tx], cH[1][ty + 1][tx + 1]));
+    const float acc = __fadd_rn(__fadd_rn(lo0, hi0), __fadd_rn(lo1, hi1));
+    out[static_cast<int64_t>(pi) * N + pj] = __fmul_rn(invK, acc);
+  }
+}
+
 // ------------------------------------------------- TV prox VJP (autodiff) --
 // Cotangents of out = tv_prox(v; tau * w) with respect to v and w, for the
 // differentiable unrolled reconstruction.  Pass 1, per padded coefficient
@@ -1192,9 +1253,15 @@ int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out,
   const int rc = tv_dims(ndim, shape, circular, t);
   if (rc) return rc;
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
-  hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
-                     as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
-                     isotropic, nullptr, pred);
+  if (ndim == 2) {
+    const dim3 g(ceil_div(t.d[2], solver::TVT), ceil_div(t.d[1], solver::TVT));
+    hipLaunchKernelGGL(solver::tv_prox2d_kernel, g, dim3(256), 0, as_stream(stream), v, out, t,
+                       tau, 0.25f, isotropic, nullptr, pred);
+  } else {
+    hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                       as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
+                       isotropic, nullptr, pred);
+  }
   SCXR_LAUNCHED("tv prox");
   return SCXR_OK;
 }
@@ -1208,9 +1275,15 @@ int scxr_tv_prox_w(int32_t ndim, const int32_t* shape, const float* v, const flo
   const int rc = tv_dims(ndim, shape, circular, t);
   if (rc) return rc;
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
-  hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
-                     as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
-                     isotropic, w, pred);
+  if (ndim == 2) {
+    const dim3 g(ceil_div(t.d[2], solver::TVT), ceil_div(t.d[1], solver::TVT));
+    hipLaunchKernelGGL(solver::tv_prox2d_kernel, g, dim3(256), 0, as_stream(stream), v, out, t,
+                       tau, 0.25f, isotropic, w, pred);
+  } else {
+    hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                       as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
+                       isotropic, w, pred);
+  }
   SCXR_LAUNCHED("tv prox w");
   return SCXR_OK;
 }
