@@ -64,8 +64,16 @@ def zeros(shape, dtype=np.float32) -> torch.Tensor:
     return torch.zeros(tuple(shape), dtype=torch_dtype(dtype), device=DEVICE)
 
 
+# torch's raw current-stream query avoids building a Stream object per call
+# (about 0.1 us instead of 3 us, which adds up over the dozens of launches
+# of one solver iteration)
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream() -> int:
     """Raw hipStream_t of torch's current stream (captured by graphs)."""
+    if _RAW_STREAM is not None:
+        return _RAW_STREAM(torch.cuda.current_device())
     return torch.cuda.current_stream().cuda_stream
 
 
