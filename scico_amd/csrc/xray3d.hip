@@ -241,10 +241,8 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
 #pragma unroll
             for (int k = 0; k < SPG; ++k) {
               const int L = first[gg + 1] + k;
-              if (L < first[gg + 2])
-                xs[c][k] = bload4(vr, qoff, (rpix + (L - base) * kstep) * rowb);
-              else
-                xs[c][k] = make_float4(0.f, 0.f, 0.f, 0.f);
+              // an empty slot's register is never read
+              if (L < first[gg + 2]) xs[c][k] = bload4(vr, qoff, (rpix + (L - base) * kstep) * rowb);
             }
           }
         }
