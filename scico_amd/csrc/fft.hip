@@ -223,21 +223,17 @@ __device__ __forceinline__ void dft4_strided(float2 (&v)[N], int o, bool inv) {
   v[o + 3 * S] = csub(d02, t);
 }
 
-// exp(-+2 pi i m / 16) for m = 0 .. 9 (forward sign; conjugated for the
-// inverse), the internal twiddles of the 8- and 16-point butterflies.
+// exp(-2 pi i m / 16) = (cos(pi m / 8), -sin(pi m / 8)) for m = 0 .. 15,
+// conjugated for the inverse: the internal twiddles of the 8- and 16-point
+// butterflies (m is a compile-time constant there, so the tables fold).
 __device__ __forceinline__ float2 w16(int m, bool inv) {
   constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f;
   constexpr float h = 0.70710678118654752f;
-  float2 w;
-  switch (m) {
-    case 0: w = make_float2(1.0f, 0.0f); break;
-    case 1: w = make_float2(c1, -s1); break;
-    case 2: w = make_float2(h, -h); break;
-    case 3: w = make_float2(s1, -c1); break;
-    case 4: w = make_float2(0.0f, -1.0f); break;
-    case 6: w = make_float2(-h, -h); break;
-    default: w = make_float2(-c1, s1); break;  // m = 9
-  }
+  constexpr float cs[16] = {1.0f, c1, h, s1, 0.0f, -s1, -h, -c1,
+                            -1.0f, -c1, -h, -s1, 0.0f, s1, h, c1};
+  constexpr float sn[16] = {0.0f, s1, h, c1, 1.0f, c1, h, s1,
+                            0.0f, -s1, -h, -c1, -1.0f, -c1, -h, -s1};
+  const float2 w = make_float2(cs[m & 15], -sn[m & 15]);
   return inv ? make_float2(w.x, -w.y) : w;
 }
 
