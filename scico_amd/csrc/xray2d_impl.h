@@ -18,25 +18,27 @@ constexpr int ADJ_VC = 32;
 constexpr int ADJ_VS = 2;  // groups of 256 threads splitting the views of a tile
 constexpr int ADJ_WFAST = 64;  // window slots (one per lane)
 
-// Forward: 4 waves x ch chains (4, or 2 for wide detectors) private LDS
-// copies of (D + 4) bins per owned view.
+// Forward: FWD_WAVES waves x ch chains (2, or 1 for wide detectors)
+// private LDS copies of (D + 4) bins per owned view.
+constexpr int FWD_WAVES = 8;
 constexpr size_t FWD_LDS_MAX = 64 * 1024;
 constexpr int FWD_SMAX = 64;  // largest lane stride of the forward projector
 inline size_t fwd_lds_bytes(int vpg, int D, int ch) {
-  return sizeof(float) * static_cast<size_t>(vpg) * 4 * ch * (static_cast<size_t>(D) + 4);
+  return sizeof(float) * static_cast<size_t>(vpg) * FWD_WAVES * ch *
+         (static_cast<size_t>(D) + 4);
 }
-// Chains per wave for vpg views of D bins: 4 when the rows fit, else 2
+// Chains per wave for vpg views of D bins: 2 when the rows fit, else 1
 // (0: too wide).
 inline int fwd_chains(int vpg, int D) {
-  if (fwd_lds_bytes(vpg, D, 4) <= FWD_LDS_MAX) return 4;
   if (fwd_lds_bytes(vpg, D, 2) <= FWD_LDS_MAX) return 2;
+  if (fwd_lds_bytes(vpg, D, 1) <= FWD_LDS_MAX) return 1;
   return 0;
 }
 
 struct Plan {
   int M = 0, N = 0, A = 0, D = 0;
   int vpg = 1;         // views per forward workgroup
-  int fwd_ch = 4;      // forward chains per wave
+  int fwd_ch = 2;      // forward chains per wave
   int n_groups = 0;    // forward workgroups
   int adj_w = 0;       // adjoint window width (bins)
   int adj_ppt = 1;     // adjoint pixels per thread
@@ -109,7 +111,7 @@ struct BatchTabs {
   int nf = 0, na = 0, nt = 0;
   int ppt = 1;                    // adjoint pixels per thread (all slices)
   int fract = 0;                  // 1: Px >= 0 for every pixel of every slice
-  int ch = 4;                     // forward chains per wave (all slices)
+  int ch = 2;                     // forward chains per wave (all slices)
 };
 
 // Epilogue arguments of the batched adjoint (packed vectors; z, u are

@@ -29,7 +29,7 @@ CASES = [
     ((256, 256), 180, {"det_count": 363}),  # 256^2, 180 views, 363 bins
     ((40, 24), 1100, {}),  # two views per forward workgroup
     ((20, 20), 2100, {"dx": 0.9}),  # four views per forward workgroup
-    ((30, 30), 7, {"det_count": 1100}),  # wide detector: two chains per wave
+    ((30, 30), 7, {"det_count": 1100}),  # wide detector: one chain per wave
     ((16, 16), 5, {"dx": 1.8, "det_count": 60}),  # one pixel per adjoint thread
     ((24, 20), 13, {"x0": (-7.0, -15.0), "dx": (0.8, 1.25)}),  # offset, anisotropic
 ]
