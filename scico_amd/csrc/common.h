@@ -62,6 +62,14 @@ __device__ __forceinline__ float div_rn(float d, double rw) {
   return static_cast<float>(static_cast<double>(d) * rw);
 }
 
+// min(d / w, 1) for d >= 0 with the quotient rounded as by div_rn.  The
+// median of (q, 0, 1) equals min(q, 1) for q >= 0, and the compiler turns
+// it into the clamp bit of the float64-to-float32 conversion, so the
+// clamp costs no instruction of its own.
+__device__ __forceinline__ float div_rn_sat(float d, double rw) {
+  return __builtin_amdgcn_fmed3f(div_rn(d, rw), 0.0f, 1.0f);
+}
+
 // floor(x) converted to int32 in one instruction (for |x| < 2^31).
 __device__ __forceinline__ int cvt_flr_i32(float x) {
   int r;

@@ -280,14 +280,14 @@ __device__ __forceinline__ Weight weight_at(float Pxmin, float P0, float P1, flo
   const float Px = __fadd_rn(__fadd_rn(Pxmin, __fmul_rn(P0, fi)), __fmul_rn(P1, fj));
   const float fl = floorf(Px);
   const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
-  return Weight{fl, fminf(div_rn(dd, rw), 1.0f)};
+  return Weight{fl, div_rn_sat(dd, rw)};
 }
 
 // The same weight from the fraction instruction, for Px >= 0, where
 // fract(Px) = Px - floor(Px) exactly (the adjoint's path when every
 // projected coordinate of the plan is non-negative).
 __device__ __forceinline__ float weight_frac(float Px, double rw) {
-  return fminf(div_rn(__fsub_rn(1.0f, __builtin_amdgcn_fractf(Px)), rw), 1.0f);
+  return div_rn_sat(__fsub_rn(1.0f, __builtin_amdgcn_fractf(Px)), rw);
 }
 
 // Transpose (M, N) -> (N, M) through a padded LDS tile (tile row by,
@@ -356,7 +356,7 @@ __device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
       for (int c = 0; c < CH; ++c) {
         const float P = COLFAST ? __fadd_rn(tl, t[c][v]) : __fadd_rn(t[c][v], tl);
         const float fl = floorf(P);
-        w[c] = fminf(div_rn(__fsub_rn(1.0f, __fsub_rn(P, fl)), rw[v]), 1.0f);
+        w[c] = div_rn_sat(__fsub_rn(1.0f, __fsub_rn(P, fl)), rw[v]);
         k[c] = row[c][v] + static_cast<int>(__builtin_amdgcn_fmed3f(fl, -2.0f, Dmax));
       }
       float a0[CH], a1[CH];
@@ -830,7 +830,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
             } else {
               const float fl = floorf(Px);
               const float dd = __fsub_rn(1.0f, __fsub_rn(Px, fl));
-              w[u][q] = fminf(div_rn(dd, tv.rw), 1.0f);
+              w[u][q] = div_rn_sat(dd, tv.rw);
               k[u][q] = static_cast<int>(
                   static_cast<unsigned>(__fmaf_rn(fl, 4.0f, __uint_as_float(tv.base4))));
             }
