@@ -136,7 +136,7 @@ int create(Batch& b, int ns, const int32_t* dims, const float* angles, const dou
     }
     b.bt.ppt = ppt;
     b.bt.fract = 1;
-    for (int s = 0; s < ns; ++s) b.bt.fract = b.bt.fract && tabs[s].px_nonneg;
+    for (int s = 0; s < ns; ++s) b.bt.fract = b.bt.fract && tabs[s].px_ge1;
   }
   {
     int Dm = 0;

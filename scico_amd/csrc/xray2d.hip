@@ -156,15 +156,15 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
     }
   }
 
-  // Px >= 0 for every pixel of every view (bounds of Px over the image in
+  // Px >= 1 for every pixel of every view (bounds of Px over the image in
   // float64, with a margin far above the float32 rounding of Px): then
-  // Px - floor(Px) is the exact fraction and the adjoint may take it from
-  // the fraction instruction.
-  h.px_nonneg = true;
+  // Px - floor(Px) is the exact fraction and the adjoint may take it, and
+  // the weight, from the fraction instruction (see weight_frac).
+  h.px_ge1 = true;
   for (int a = 0; a < A; ++a) {
     const double lo = static_cast<double>(geom[a].x) + std::min<double>(geom[a].y, 0.0) * (M - 1) +
                       std::min<double>(geom[a].z, 0.0) * (N - 1);
-    h.px_nonneg = h.px_nonneg && lo >= 0.01;
+    h.px_ge1 = h.px_ge1 && lo >= 1.01;
   }
 
   // Adjoint: window origin offset per view (see adj_tile): a lower bound
@@ -216,7 +216,7 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg, 0);
   if (rc) return rc;
   p.adj_ppt = h.adj_ppt;
-  p.adj_fract = h.px_nonneg;
+  p.adj_fract = h.px_ge1;
   p.host_geom = h.geom;
   p.any_colfast = h.any_colfast;
   p.n_groups = static_cast<int>(h.groups.size());
@@ -283,11 +283,15 @@ __device__ __forceinline__ Weight weight_at(float Pxmin, float P0, float P1, flo
   return Weight{fl, div_rn_sat(dd, rw)};
 }
 
-// The same weight from the fraction instruction, for Px >= 0, where
-// fract(Px) = Px - floor(Px) exactly (the adjoint's path when every
-// projected coordinate of the plan is non-negative).
+// The same weight from the fraction instruction, for Px >= 1: there
+// f = fract(Px) = Px - floor(Px) exactly, and 1 - f is exact in float32 as
+// well (a multiple of ulp(Px) >= 2^-23 in (0, 1]), so the float64
+// multiply-add rw - f rw rounds the product (1 - f) rw once, exactly as
+// div_rn(1 - f, rw) does (the adjoint's path when every
+// projected coordinate of the plan is at least 1).
 __device__ __forceinline__ float weight_frac(float Px, double rw) {
-  return div_rn_sat(__fsub_rn(1.0f, __builtin_amdgcn_fractf(Px)), rw);
+  const double f = static_cast<double>(__builtin_amdgcn_fractf(Px));
+  return __builtin_amdgcn_fmed3f(static_cast<float>(fma(-f, rw, rw)), 0.0f, 1.0f);
 }
 
 // Transpose (M, N) -> (N, M) through a padded LDS tile (tile row by,
@@ -952,7 +956,7 @@ __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds
   const size_t p = e - static_cast<size_t>(a) * n;
   const int i = static_cast<int>(p / N), j = static_cast<int>(p - static_cast<size_t>(i) * N);
   const float4 g = geom[a];
-  if (fract) {  // the adjoint's arithmetic for plans with Px >= 0 everywhere
+  if (fract) {  // the adjoint's arithmetic for plans with Px >= 1 everywhere
     const float Px = __fadd_rn(__fadd_rn(g.x, __fmul_rn(g.y, static_cast<float>(i))),
                                __fmul_rn(g.z, static_cast<float>(j)));
     inds[e] = cvt_flr_i32(Px);

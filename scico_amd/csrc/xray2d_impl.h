@@ -42,7 +42,7 @@ struct Plan {
   int n_groups = 0;    // forward workgroups
   int adj_w = 0;       // adjoint window width (bins)
   int adj_ppt = 1;     // adjoint pixels per thread
-  bool adj_fract = false;  // Px >= 0 everywhere: fraction-instruction weights
+  bool adj_fract = false;  // Px >= 1 everywhere: fraction-instruction weights
   bool any_colfast = false;
   void* dev_block = nullptr;  // device tables (one allocation)
   float4* geom = nullptr;     // (Pxmin, P0, P1, width) per view
@@ -79,7 +79,7 @@ struct HostTables {
   int adj_w = 0;
   int adj_ppt = 1;
   bool any_colfast = false;
-  bool px_nonneg = false;  // Px >= 0 for every pixel and view
+  bool px_ge1 = false;  // Px >= 1 for every pixel and view
 };
 // vpg: views per forward group; ppt: adjoint pixels per thread (0 = 2 when
 // the adjoint window fits, else 1).
@@ -110,7 +110,7 @@ struct BatchTabs {
   const int4* titems = nullptr;   // transpose items (slice, tile row, tile col, 0)
   int nf = 0, na = 0, nt = 0;
   int ppt = 1;                    // adjoint pixels per thread (all slices)
-  int fract = 0;                  // 1: Px >= 0 for every pixel of every slice
+  int fract = 0;                  // 1: Px >= 1 for every pixel of every slice
   int ch = 2;                     // forward chains per wave (all slices)
 };
 
