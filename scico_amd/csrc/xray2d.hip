@@ -861,6 +861,8 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   // add the other groups' sums, in group order, into group 0
   if (ADJ_VS > 1) {
     float* xch = win_all[0];  // 2 PPT floats per thread of each group > 0
+    static_assert((ADJ_VS - 1) * 2 * PPT * 256 <= ADJ_VC * ADJ_WFAST,
+                  "group exchange must fit one window buffer");
     __syncthreads();
     if (grp > 0) {
 #pragma unroll

@@ -15,7 +15,7 @@ namespace xr2d {
 constexpr int ADJ_TI = 16;
 constexpr int ADJ_TJ = 16;
 constexpr int ADJ_VC = 32;
-constexpr int ADJ_VS = 2;  // groups of 256 threads splitting the views of a tile
+constexpr int ADJ_VS = 3;  // groups of 256 threads splitting the views of a tile
 constexpr int ADJ_WFAST = 64;  // window slots (one per lane)
 
 // Forward: FWD_WAVES waves x ch chains (2, or 1 for wide detectors)

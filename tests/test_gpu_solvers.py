@@ -176,7 +176,8 @@ def test_admm_tv_one_step(gpu, N, A_, D, circular, iso):
 
 def test_admm_tv_multi_step_noise_floor(gpu):
     """Six iterations: GPU vs oracle within 10x the oracle's drift under +-1
-    ulp projector noise; reconstruction SNR within 0.1 dB."""
+    ulp projector noise; reconstruction SNR within the SNR drift of that
+    noisy oracle run (0.26 dB on this problem), at least 0.1 dB."""
     N, A_, D = 48, 60, 68
     ang, g, xgt, y = _problem(N, A_, D)
     xr, hr = _oracle_admm(g, y, 2e-2, 1.0, 6, 10)
@@ -186,10 +187,12 @@ def test_admm_tv_multi_step_noise_floor(gpu):
     floor = rel_l2(xn, xr)
     xf, hf = _gpu_admm(ang, y, N, D, 2e-2, 1.0, 6, 10)
     e = rel_l2(xf, xr)
+    snr_floor = abs(_snr(xn, xgt) - _snr(xr, xgt))
     print(f"  6-step ADMM: GPU vs oracle {e:.2e}; ulp-noise floor {floor:.2e}; "
-          f"SNR gpu {_snr(xf, xgt):.4f} oracle {_snr(xr, xgt):.4f} dB", flush=True)
+          f"SNR gpu {_snr(xf, xgt):.4f} oracle {_snr(xr, xgt):.4f} "
+          f"noisy oracle {_snr(xn, xgt):.4f} dB", flush=True)
     assert e <= max(10 * floor, 1e-5)
-    assert abs(_snr(xf, xgt) - _snr(xr, xgt)) < 0.1
+    assert abs(_snr(xf, xgt) - _snr(xr, xgt)) <= max(snr_floor, 0.1)
     assert list(hf.CG_It) == [h[4] for h in hr]
 
 
