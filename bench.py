@@ -439,7 +439,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=18)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--kernels", action="store_true",
                     help="projector launches only (for counter runs)")
