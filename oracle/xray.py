@@ -264,7 +264,7 @@ def fbp(y, x0, dx, nx, y0, angles):
     L = 2 * N - 1
     hf = np.fft.fft(h.astype(np.complex64), n=L, axis=1)
     yf = np.fft.fft(y.astype(np.complex64), n=L, axis=1)
-    hy = np.fft.ifft(hf * yf, n=L, axis=1)[:, (N - 1) // 2 : -((N - 1) // 2)].real.astype(f32)
+    hy = np.fft.ifft(hf * yf, n=L, axis=1)[:, (N - 1) // 2 : -(N - 1) // 2].real.astype(f32)  # :192-194
     scale = f32(np.pi * float(dx[0]) * float(dx[1]) / A)
     return (scale * mask * back_project(hy, x0, dx, nx, y0, angles)).astype(f32)
 

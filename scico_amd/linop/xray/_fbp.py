@@ -19,19 +19,32 @@ def ramp_filter(x, tau: float) -> np.ndarray:
     return out
 
 
+def filter_window(N: int) -> tuple[int, int]:
+    """First index and length of the filtered row within the full linear
+    convolution of length ``2 N - 1``.  The reference slices
+    ``[(N - 1) // 2 : -(N - 1) // 2]``, where ``-(N - 1) // 2`` is
+    ``floor(-(N - 1) / 2)``, so the row keeps exactly ``N`` bins for every
+    ``N`` (``_xray2d.py:192-194``)."""
+    r = range(2 * N - 1)[(N - 1) // 2 : -(N - 1) // 2]
+    if len(r) != N:  # N = 1: the slice [0:0] is empty
+        raise ValueError(f"fbp needs at least 2 detector bins, got {N}")
+    return r.start, len(r)
+
+
 def fbp(H, y):
     """``x = (pi dx0 dx1 / A) * mask * H^T(ramp * y)``.
 
     The rows are filtered by direct linear convolution (same result as the
-    reference's zero-padded FFT, up to float32 rounding).  For even
-    ``det_count`` the reference's slice keeps ``det_count + 1`` filtered
-    bins; that row length is reproduced, and back projected with the same
-    geometry over ``det_count + 1`` bins.
+    reference's zero-padded FFT, up to float32 rounding), keeping the
+    window of the full convolution that the reference's slice
+    ``[(N - 1) // 2 : -(N - 1) // 2]`` keeps (``_xray2d.py:192-194``).
     """
     y = dev.as_device(y)
+    if tuple(y.shape) != tuple(H.output_shape):
+        raise ValueError(f"Shapes do not conform: input array shape {tuple(y.shape)} != "
+                         f"output_shape {tuple(H.output_shape)}")
     A, N = y.shape
-    s = (N - 1) // 2
-    Dout = (2 * N - 1) - 2 * s
+    s, Dout = filter_window(N)
     if H.fbp_filter is None:
         H.fbp_filter = dev.as_device(ramp_filter(np.arange(N) - s, 1.0))
     if H.fbp_mask is None:
@@ -40,16 +53,6 @@ def fbp(H, y):
     hy = dev.empty((A, Dout))
     call("scxr_ramp_filter", A, N, Dout, s, H.fbp_filter.data_ptr(), y.data_ptr(),
          hy.data_ptr(), None, dev.stream())
-    if Dout == N:
-        bp = H.back_project(hy)
-    else:
-        Hx = getattr(H, "_fbp_wide", None)
-        if Hx is None:
-            from ._xray2d import XRayTransform2D
-
-            Hx = XRayTransform2D(H.input_shape, H.angles, x0=H.x0, dx=H.dx, y0=H.y0,
-                                 det_count=Dout)
-            H._fbp_wide = Hx
-        bp = Hx.back_project(hy)
+    bp = H.back_project(hy)
     c = float(np.float32(np.pi * float(np.asarray(H.dx[0])) * float(np.asarray(H.dx[1])) / A))
     return vec.mul(vec.scale(c, H.fbp_mask), bp)

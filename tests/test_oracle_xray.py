@@ -181,3 +181,22 @@ def test_bin_flip_rates(N, A, step):
     r_fma, r_lib = _flip_rates(N, A, step)
     print(f"\n  {N}^2/{A} views: bin flips with FMA {r_fma:.2e}, with float32 cos/sin {r_lib:.2e}")
     assert r_fma < 1e-3 and r_lib < 1e-2
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 64, 91, 184, 363, 724])
+def test_fbp_filter_window_keeps_n_bins(N):
+    """The reference's slice [(N-1)//2 : -(N-1)//2] of the length 2N-1
+    convolution (_xray2d.py:192-194) keeps N bins for even and odd N; the
+    product's window and the oracle's filtered rows have that length."""
+    from scico_amd.linop.xray._fbp import filter_window
+
+    s, L = filter_window(N)
+    assert L == N and s == (N - 1) // 2
+    full = np.arange(2 * N - 1)
+    assert np.array_equal(full[s : s + L], full[(N - 1) // 2 : -(N - 1) // 2])
+    ang = np.linspace(0, np.pi, 3, endpoint=False)
+    g = geom((8, 8), ang, det_count=N)
+    y = np.ones((3, N), np.float32)
+    # the oracle's FBP back projects the filtered rows with the N-bin geometry
+    xr = X.fbp(y, g["x0"], g["dx"], g["nx"], g["y0"], g["angles"])
+    assert xr.shape == (8, 8)
