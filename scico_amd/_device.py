@@ -77,6 +77,50 @@ def stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
+def input_array(x, shape, what: str = "input", dtype=np.float32) -> torch.Tensor:
+    """``as_device(x, dtype)`` checked to have exactly ``shape`` before its
+    pointer goes to the C ABI (which trusts sizes).  Raises ``ValueError``
+    as scico does for non-conforming shapes (``_operator.py:219-223``)."""
+    t = as_device(x, dtype)
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"Shapes do not conform: {what} array shape {tuple(t.shape)} != "
+                         f"expected shape {tuple(shape)}")
+    return t
+
+
+def output_buffer(out, shape, what: str = "out", dtype=np.float32) -> torch.Tensor:
+    """A new device array of ``shape``, or the caller's ``out`` once it is
+    checked to be a contiguous device tensor of ``dtype`` holding exactly
+    ``prod(shape)`` elements (a wrong buffer would be written out of
+    bounds by the kernels, so it is refused with ``ValueError``)."""
+    if out is None:
+        return empty(shape, dtype)
+    n = int(np.prod(shape, dtype=np.int64))
+    if not isinstance(out, torch.Tensor) or out.device.type != "cuda":
+        raise ValueError(f"{what} must be a device tensor")
+    if out.dtype != torch_dtype(dtype):
+        raise ValueError(f"{what} has dtype {out.dtype}, expected {torch_dtype(dtype)}")
+    if not out.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    if out.numel() != n:
+        raise ValueError(f"{what} holds {out.numel()} elements, expected {n} (shape {tuple(shape)})")
+    return out
+
+
+def check_buffer(t, n: int, what: str, dtype=np.float32, allow_none: bool = False,
+                 at_least: bool = False):
+    """Check a flat device operand of ``n`` elements (``at_least``: a
+    workspace of at least ``n``) and return it; ``None`` passes when
+    ``allow_none``."""
+    if t is None:
+        if allow_none:
+            return None
+        raise ValueError(f"{what} is required")
+    if at_least and isinstance(t, torch.Tensor) and t.numel() >= n:
+        return output_buffer(t, (t.numel(),), what, dtype)
+    return output_buffer(t, (int(n),), what, dtype)
+
+
 def ptr(t: torch.Tensor | None):
     return None if t is None else t.data_ptr()
 

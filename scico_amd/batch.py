@@ -72,6 +72,8 @@ class XRayTransform2DBatch:
         self.output_shapes = [(len(sl[1]), sl[5]) for sl in self.slices]
         self.vpg = int(vpg)
         self._plan = None
+        self._retired = []
+        self.generation = 0
         self.config = (0, 0, 0)
         self._create()
 
@@ -89,9 +91,12 @@ class XRayTransform2DBatch:
              ang.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
              geo.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.vpg, ctypes.byref(h))
         if self._plan is not None:
-            self._finalizer()
+            # A graph captured from the old plan still reads its tables:
+            # keep them until this object goes away instead of freeing them.
+            self._retired.append(self._plan)
         self._plan = h
-        self._finalizer = weakref.finalize(self, lib.scxr_xr2d_batch_destroy, h)
+        self.generation += 1
+        weakref.finalize(self, lib.scxr_xr2d_batch_destroy, h)
         info = (ctypes.c_int64 * 11)()
         call("scxr_xr2d_batch_info", h, info)
         (_, self.n_img, self.n_sino, self.n_fwd_items, self.n_adj_items, self.n_t_items,
@@ -135,15 +140,41 @@ class XRayTransform2DBatch:
                 for s, (A, D) in enumerate(self.output_shapes)]
 
     # --- projections of the packed batch
+    def _check_pred(self, pred, pstride):
+        if pred is not None:
+            need = (self.ns - 1) * int(pstride) + 1
+            dev.check_buffer(pred, need, "pred", np.int32, at_least=True)
+
     def project(self, im, out=None, y=None, res_ws=None, pred=None, pstride: int = 1):
-        out = dev.empty((self.n_sino,)) if out is None else out
+        """Forward projection of the packed images ``im`` (``n_img``) into
+        packed sinograms (``n_sino``); with ``y`` and ``res_ws`` also the
+        forward-group partials of ``||y - A im||^2``."""
+        dev.check_buffer(im, self.n_img, "im")
+        out = dev.output_buffer(out, (self.n_sino,))
+        dev.check_buffer(y, self.n_sino, "y", allow_none=True)
+        if (y is None) != (res_ws is None):
+            raise ValueError("y and res_ws go together")
+        dev.check_buffer(res_ws, self.n_groups, "res_ws", np.float64, allow_none=True,
+                         at_least=True)
+        self._check_pred(pred, pstride)
         call("scxr_xr2d_batch_forward", self._plan, im.data_ptr(), out.data_ptr(), dev.ptr(y),
              dev.ptr(res_ws), dev.ptr(pred), int(pstride), dev.stream())
         return out
 
     def back_project(self, sino, out=None, kind=0, rho=0.0, circular=False, p=None, z=None,
                      u=None, dot_ws=None, pred=None, pstride: int = 1):
-        out = dev.empty((self.n_img,)) if out is None else out
+        """Back projection of packed sinograms with the epilogues of
+        ``scxr_xr2d_adjoint_epi`` per slice (``kind`` 0, 1 or 2)."""
+        dev.check_buffer(sino, self.n_sino, "sino")
+        out = dev.output_buffer(out, (self.n_img,))
+        if kind not in (0, 1, 2):
+            raise ValueError(f"kind must be 0, 1 or 2, got {kind}")
+        dev.check_buffer(p, self.n_img, "p", allow_none=kind != 1)
+        dev.check_buffer(z, 2 * self.n_img, "z", allow_none=kind != 2)
+        dev.check_buffer(u, 2 * self.n_img, "u", allow_none=True)
+        dev.check_buffer(dot_ws, self.n_tiles, "dot_ws", np.float64, allow_none=True,
+                         at_least=True)
+        self._check_pred(pred, pstride)
         call("scxr_xr2d_batch_adjoint", self._plan, sino.data_ptr(), out.data_ptr(), int(kind),
              float(rho), int(circular), dev.ptr(p), dev.ptr(z), dev.ptr(u), dev.ptr(dot_ws),
              dev.ptr(pred), int(pstride), dev.stream())
@@ -214,6 +245,7 @@ class ADMMTVBatch:
                  cg_maxiter: int = 10, cg_tol: float = 1e-4, cg_atol: float = 0.0,
                  isotropic: bool = True, circular: bool = False, hist_cap: int = 64):
         self.H = H
+        self.generation = H.generation
         self.y = y if isinstance(y, torch.Tensor) else H.pack_sinograms(y)
         self.lam, self.rho = float(lam), float(rho)
         self.iso, self.circ = bool(isotropic), int(bool(circular))
@@ -251,8 +283,14 @@ class ADMMTVBatch:
         vec.fill(self.counters.view(torch.float32), 0.0)
 
     def enqueue(self):
-        """One ADMM iteration on every slice (no host synchronisation)."""
+        """One ADMM iteration on every slice (no host synchronisation).
+
+        The workspaces are sized for the plan ``H`` had when this solver was
+        built; if ``H`` was re-created since (``autotune``), this raises."""
         H, P, s = self.H, (lambda t: t.data_ptr()), dev.stream()
+        if H.generation != self.generation:
+            raise RuntimeError("the batch plan was re-created (autotune) after this solver was "
+                               "built; build a new ADMMTVBatch")
         F = self.maxiter + 1
         H.back_project(self.y, out=self.rhs, kind=2, rho=self.rho, circular=self.circ, z=self.z,
                        u=self.u)

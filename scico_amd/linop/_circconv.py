@@ -112,7 +112,8 @@ class CircularConvolve(LinearOperator):
         return out
 
     def _apply(self, x, adjoint, out=None, pred=None):
-        out = dev.empty(self.input_shape) if out is None else out
+        x = dev.input_array(x, self.input_shape, "x")
+        out = dev.output_buffer(out, self.input_shape)
         call("scxr_cconv_apply", self.plan, x.data_ptr(), out.data_ptr(), int(adjoint),
              dev.ptr(_pred.resolve(pred)), dev.stream())
         return out
@@ -127,6 +128,9 @@ class CircularConvolve(LinearOperator):
         """Complex fftn (ifftn) of ``x`` over the convolution axes."""
         x = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x))
         x = x.to(dev.DEVICE).contiguous()
+        if tuple(x.shape) != tuple(self.input_shape):
+            raise ValueError(f"Shapes do not conform: x array shape {tuple(x.shape)} != "
+                             f"expected shape {tuple(self.input_shape)}")
         real = not x.is_complex()
         if real:
             x = x.to(torch.float32)

@@ -56,13 +56,15 @@ class FiniteDifference(LinearOperator):
                          input_dtype=np.float32, output_dtype=np.float32)
 
     def _eval(self, x, out=None, pred=None):
-        out = dev.empty(self.output_shape) if out is None else out
+        x = dev.input_array(x, self.input_shape, "x")
+        out = dev.output_buffer(out, self.output_shape)
         call("scxr_fd_forward", len(self.input_shape), self._shape_c, x.data_ptr(),
              out.data_ptr(), int(self.circular), dev.ptr(_pred.resolve(pred)), dev.stream())
         return out
 
     def _adj(self, g, out=None, scale=1.0, pred=None):
-        out = dev.empty(self.input_shape) if out is None else out
+        g = dev.input_array(g, self.output_shape, "g")
+        out = dev.output_buffer(out, self.input_shape)
         call("scxr_fd_adjoint", len(self.input_shape), self._shape_c, g.data_ptr(),
              out.data_ptr(), int(self.circular), float(scale), dev.ptr(_pred.resolve(pred)),
              dev.stream())
@@ -70,8 +72,8 @@ class FiniteDifference(LinearOperator):
 
     def gram(self, x, rho=1.0, out=None, pred=None):
         """``rho * C^T C x`` in one stencil pass."""
-        x = dev.as_device(x)
-        out = dev.empty(self.input_shape) if out is None else out
+        x = dev.input_array(x, self.input_shape, "x")
+        out = dev.output_buffer(out, self.input_shape)
         call("scxr_fd_gram", len(self.input_shape), self._shape_c, float(rho), x.data_ptr(),
              out.data_ptr(), int(self.circular), dev.ptr(_pred.resolve(pred)), dev.stream())
         return out

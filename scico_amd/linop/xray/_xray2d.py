@@ -126,16 +126,16 @@ class XRayTransform2D(LinearOperator):
 
     def project(self, im, out=None, pred=None):
         """X-ray projection ``H @ im`` (``_xray2d.py:138-146``)."""
-        im = dev.as_device(im)
-        out = dev.empty(self.output_shape) if out is None else out
+        im = dev.input_array(im, self.input_shape, "im")
+        out = dev.output_buffer(out, self.output_shape)
         call("scxr_xr2d_forward", self.plan, im.data_ptr(), out.data_ptr(), dev.ptr(_pred.resolve(pred)),
              dev.stream())
         return out
 
     def back_project(self, y, out=None, pred=None):
         """Back projection ``H.T @ y`` (``_xray2d.py:148-156``)."""
-        y = dev.as_device(y)
-        out = dev.empty(self.input_shape) if out is None else out
+        y = dev.input_array(y, self.output_shape, "y")
+        out = dev.output_buffer(out, self.input_shape)
         call("scxr_xr2d_adjoint", self.plan, y.data_ptr(), out.data_ptr(), dev.ptr(_pred.resolve(pred)),
              dev.stream())
         return out

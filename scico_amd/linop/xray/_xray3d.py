@@ -91,16 +91,16 @@ class XRayTransform3D(LinearOperator):
 
     def project(self, im, out=None, pred=None):
         """X-ray projection (``_xray3d.py:95-100``)."""
-        im = dev.as_device(im)
-        out = dev.empty(self.output_shape) if out is None else out
+        im = dev.input_array(im, self.input_shape, "im")
+        out = dev.output_buffer(out, self.output_shape)
         call("scxr_xr3d_forward", self.plan, im.data_ptr(), out.data_ptr(),
              dev.ptr(_pred.resolve(pred)), dev.stream())
         return out
 
     def back_project(self, proj, out=None, pred=None):
         """Back projection (``_xray3d.py:102-108``)."""
-        proj = dev.as_device(proj)
-        out = dev.empty(self.input_shape) if out is None else out
+        proj = dev.input_array(proj, self.output_shape, "proj")
+        out = dev.output_buffer(out, self.input_shape)
         call("scxr_xr3d_adjoint", self.plan, proj.data_ptr(), out.data_ptr(),
              dev.ptr(_pred.resolve(pred)), dev.stream())
         return out
@@ -108,6 +108,8 @@ class XRayTransform3D(LinearOperator):
     def weights(self, view: int):
         """Debug: ``(ul (2, X, Y, Z) int32, w (4, X, Y, Z))`` of one view,
         as ``_calc_weights`` returns them (``_xray3d.py:206-266``)."""
+        if not 0 <= int(view) < self.output_shape[0]:
+            raise ValueError(f"view {view} out of range [0, {self.output_shape[0]})")
         ul = dev.empty((2, *self.input_shape), dtype=np.int32)
         w = dev.empty((4, *self.input_shape))
         call("scxr_xr3d_weights", self.plan, int(view), ul.data_ptr(), w.data_ptr(), dev.stream())
