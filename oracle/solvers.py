@@ -399,6 +399,59 @@ def sql2_identity_prox(v, lam, y, scale=0.5, dtype=f32):
     return ((c * np.asarray(y, dtype) + np.asarray(v, dtype)) / (c + dtype(1))).astype(dtype)
 
 
+# ---------------------------------------------------------------- phantoms --
+
+
+def circular_phantom(shape, radius_list, val_list):
+    """``scico.examples.create_circular_phantom`` (``examples.py:384-436``):
+    concentric discs about the array centre ``(dim - 1) / 2``, later
+    entries overwriting earlier ones where ``dist < r`` (float64)."""
+    center = [(d - 1) / 2 for d in shape]
+    mesh = np.meshgrid(*[np.arange(d) for d in shape], sparse=True, indexing="ij")
+    dist = np.sqrt(sum((mesh[i] - center[i]) ** 2 for i in range(len(shape))))
+    img = np.zeros(shape)
+    for r, v in zip(radius_list, val_list):
+        img[dist < r] = v
+    return img
+
+
+def tv_denoise_problem(ndim):
+    """``(x_gt, y)`` of the reference's TV denoising tests: ``test_tvnorm.py:61-72``
+    (ndim 2: 32^2 circular phantom plus a ramp) and ``:26-34`` (ndim 1:
+    128-sample clipped sine), noise sigma 0.02.  The reference draws the
+    noise from ``jax.random`` (absent here); this draws it from NumPy's
+    generator, seed 0."""
+    if ndim == 2:
+        N = 32
+        x_gt = circular_phantom((N, N), [0.6 * N, 0.4 * N, 0.2 * N, 0.1 * N],
+                                [0.25, 1, 0, 0.5]).astype(f32)
+        gr, gc = np.ogrid[0:N, 0:N]
+        x_gt += ((gr + gc) / (4 * N)).astype(f32)
+    else:
+        g = np.linspace(0, 2 * np.pi, 128, dtype=f32)
+        x_gt = np.sin(2 * g)
+        x_gt[x_gt > 0.5] = 0.5
+        x_gt[x_gt < -0.5] = -0.5
+    noise = np.random.default_rng(0).standard_normal(x_gt.shape).astype(f32)
+    return x_gt, (x_gt + f32(0.02) * noise).astype(f32)
+
+
+def tv_denoise_pair(y, lam, isotropic, circular, admm_its, L0, apgm_its, rho=1e1,
+                    thr_factor=1.0):
+    """The two solutions the reference's TV tests compare
+    (``test_tvnorm.py:74-120``): ADMM (x0 = y, CG tol 1e-4 / maxiter 25)
+    with ``lam * L21 | L1`` of ``FiniteDifference``, and AcceleratedPGM
+    (x0 = y, constant ``L0``) with the Haar TV prox.  ``thr_factor`` scales
+    the prox threshold (1 for the reference)."""
+    I = lambda v: v
+    x_admm, hist = admm_tv(y, I, I, lam, rho, y, admm_its, cg_tol=1e-4, cg_maxiter=25,
+                           circular=circular, isotropic=isotropic)
+    grad = lambda v: (v - y).astype(f32)
+    prox = lambda v, s: tv_prox(v, thr_factor * lam * s, isotropic=isotropic, circular=circular)
+    x_apgm, _ = apgm(grad, prox, L0, y, apgm_its)
+    return x_admm, x_apgm, hist
+
+
 # ------------------------------------------------------- CircularConvolve --
 
 

@@ -12,7 +12,8 @@ the same iteration runs through fused kernels (:class:`_FusedTV`): the
 x-step operator ``rho C^T C + A^T A`` and the right-hand side are fused
 into the back projection, the CG scalars and stopping test stay on the
 device, and the z/u update and iteration statistics are single passes.
-With ``XRayTransform3D`` (3D, FiniteDifference over all axes) the same
+With ``XRayTransform3D`` (3D, FiniteDifference over all axes), or the
+identity of a denoising loss ``SquaredL2Loss(y)`` (1D to 3D), the same
 structure runs with the FD terms in separate nd passes.
 """
 
@@ -28,7 +29,7 @@ from .. import _device as dev
 from .. import _vec as vec
 from .._lib import call, load
 from ..functional import L1Norm, L21Norm, ScaledFunctional
-from ..linop import FiniteDifference
+from ..linop import FiniteDifference, Identity
 from ..linop.xray import XRayTransform2D, XRayTransform3D
 from ..loss import SquaredL2Loss
 from ..solver import CGWork
@@ -177,8 +178,8 @@ class _FusedTV:
         if type(ss) is not LinearSubproblemSolver or ss.cg_function != "scico":
             return None
         f = admm.f
-        if not (isinstance(f, SquaredL2Loss) and isinstance(f.A, (XRayTransform2D, XRayTransform3D))
-                and f.scale == 0.5):
+        if not (isinstance(f, SquaredL2Loss) and f.scale == 0.5 and
+                (isinstance(f.A, (XRayTransform2D, XRayTransform3D)) or type(f.A) is Identity)):
             return None
         if len(admm.C_list) != 1 or admm.alpha != 1.0:
             return None
@@ -202,6 +203,12 @@ class _FusedTV:
         self.circ = int(self.C.circular)
         # 2D projector: FD terms fused into the projector epilogues
         self.epi = isinstance(self.A, XRayTransform2D)
+        # forward / back projection with out= and pred= (for the identity
+        # of a denoising loss, a predicated copy)
+        if type(self.A) is Identity:
+            self._fwd = self._bwd = lambda v, out, pred=None: vec.copy(v, out=out, pred=pred)
+        else:
+            self._fwd, self._bwd = self.A.project, self.A.back_project
         cgk = admm.subproblem_solver.cg_kwargs
         self.tol = float(cgk.get("tol", 1e-4))
         self.atol = float(cgk.get("atol", 0.0))
@@ -337,19 +344,19 @@ class _FusedTV:
         A, s, nd, sh = self.A, dev.stream(), self.ndim, self.shape_c
         P = lambda t: t.data_ptr()
         y = self.admm.f.y
-        A.back_project(y, out=self.rhs)
+        self._bwd(y, out=self.rhs)
         call("scxr_admm_rhs_nd", nd, sh, P(self.rhs), P(self.z), P(self.u), self.rho, self.circ,
              P(self.rhs), s)
-        A.project(self.x, out=self.sino)
-        A.back_project(self.sino, out=self.Ap)
+        self._fwd(self.x, out=self.sino)
+        self._bwd(self.sino, out=self.Ap)
         call("scxr_admm_lhs_nd", nd, sh, P(self.x), P(self.Ap), self.rho, self.circ, P(self.Ap),
              None, None, s)
         cg = self.cg
         cg.start(self.rhs, self.Ap, self.r, self.p, self.tol, self.atol)
         for k in range(self.cg_maxiter):
             flag = cg.flags[k : k + 1]
-            A.project(self.p, out=self.sino, pred=flag)
-            A.back_project(self.sino, out=self.Ap, pred=flag)
+            self._fwd(self.p, out=self.sino, pred=flag)
+            self._bwd(self.sino, out=self.Ap, pred=flag)
             call("scxr_admm_lhs_nd", nd, sh, P(self.p), P(self.Ap), self.rho, self.circ,
                  P(self.Ap), P(self.pap), P(flag), s)
             cg.iterate(k, self.x, self.r, self.p, self.Ap, self.pap, self.n_adj)
@@ -357,6 +364,6 @@ class _FusedTV:
              self.circ, int(self.iso), P(self.ws_prml), P(self.ws_g), s)
         call("scxr_admm_dual_nd", nd, sh, P(self.z), P(self.zold), self.rho, self.circ,
              P(self.ws_dual), s)
-        A.project(self.x, out=self.sino)
+        self._fwd(self.x, out=self.sino)
         vec.reduce(vec.RED_SQDIST, self.sino, y, out=self.ws_res[0])
         self._itstat()

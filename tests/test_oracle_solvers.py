@@ -81,26 +81,46 @@ def test_tv_value_identity():
     assert abs(S.tv_norm(x) - np.sum(np.sqrt(g[0].astype(np.float64) ** 2 + g[1] ** 2))) < 1e-6
 
 
+def _snr(ref, cmp):
+    """``scico.metric.snr`` (``metric.py:48-62``) in float64."""
+    ref = np.asarray(ref, np.float64)
+    return 10 * np.log10(np.var(ref) / np.mean((ref - np.asarray(cmp, np.float64)) ** 2))
+
+
 @pytest.mark.parametrize("isotropic", [True, False])
 @pytest.mark.parametrize("circular", [False, True])
-def test_tv_prox_vs_admm_denoise(isotropic, circular):
-    """test_tvnorm.py:26-181 pattern: ADMM-TV denoising (exact TV) and
-    APGM with the approximate TV prox agree (SNR > 50 dB)."""
-    N = 24
-    rng = np.random.default_rng(4)
-    xgt = np.zeros((N, N), np.float32)
-    xgt[6:18, 6:18] = 1.0
-    xgt[9:14, 3:20] += 0.5
-    y = (xgt + 0.05 * rng.standard_normal(xgt.shape)).astype(np.float32)
+def test_tv_admm_apgm_reference_pin_2d(isotropic, circular):
+    """``test_tvnorm.py:74-120`` at the reference's own bar: ADMM (rho 10,
+    x0 = y, 150 iterations, CG tol 1e-4 / maxiter 25) with lam * L21 (L1)
+    of FiniteDifference, and AcceleratedPGM (L0 1e3, 400 iterations) with
+    the approximate Haar TV prox, agree to SNR > 50 dB; and the TV value
+    equals lam * norm(FD(x)) to rel_res < 1e-6."""
+    _, y = S.tv_denoise_problem(2)
     lam = 5e-2
-    I = lambda v: v
-    x_admm, _ = S.admm_tv(y, I, I, lam, 1e1, y, 200, cg_tol=1e-6, cg_maxiter=50,
-                          circular=circular, isotropic=isotropic)
-    grad = lambda v: (v - y).astype(np.float32)
-    prox = lambda v, s: S.tv_prox(v, lam * s, isotropic=isotropic, circular=circular)
-    x_apgm, _ = S.apgm(grad, prox, 5e2, y, 1500)
-    snr = 10 * np.log10(np.var(x_admm) / np.var(x_admm - x_apgm))
-    assert snr > 30  # the reference's SNR > 50 dB uses longer runs
+    x_admm, x_apgm, _ = S.tv_denoise_pair(y, lam, isotropic, circular, 150, 1e3, 400)
+    assert _snr(x_admm, x_apgm) > 50
+    g = S.fd(x_admm, circular)
+    gv = lam * (S.l21_norm(g) if isotropic else S.l1_norm(g))
+    hv = lam * S.tv_norm(x_admm, isotropic, circular)
+    assert abs(gv - hv) / max(abs(gv), abs(hv)) < 1e-6
+
+
+@pytest.mark.parametrize("circular", [True, False])
+def test_tv_admm_apgm_reference_pin_1d(circular):
+    """``test_tvnorm.py:26-58``: 1D anisotropic TV, ADMM 50 iterations
+    against APGM (L0 5e2, 100 iterations), SNR > 50 dB."""
+    _, y = S.tv_denoise_problem(1)
+    x_admm, x_apgm, _ = S.tv_denoise_pair(y, 5e-2, False, circular, 50, 5e2, 100)
+    assert _snr(x_admm, x_apgm) > 50
+
+
+@pytest.mark.parametrize("factor", [1 / np.sqrt(2.0), 1.1])
+def test_tv_pin_catches_a_threshold_slip(factor):
+    """The 50 dB pin is sharp: a Haar prox whose threshold misses its
+    sqrt(2) factor (36 dB), or is off by 10 % (43 dB), fails it."""
+    _, y = S.tv_denoise_problem(2)
+    x_admm, x_apgm, _ = S.tv_denoise_pair(y, 5e-2, True, False, 150, 1e3, 400, thr_factor=factor)
+    assert _snr(x_admm, x_apgm) < 48
 
 
 def test_cg_direct_solve():
