@@ -162,6 +162,18 @@ int scxr_vec_div_dev(int64_t n, const float* x, const double* d, float* z, const
 int scxr_vec_reduce(int32_t kind, int64_t n, int32_t K, const float* x, const float* y,
                     double scale, int32_t post, double* out, double* ws, const int32_t* pred,
                     void* stream);
+/* scxr_vec_reduce without the ws argument; allocates nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The first-pass partials live in a static
+ * device workspace of 1024 float64 (8 KiB per device, held until the
+ * process ends; any n fits, since the first pass has at most 1024
+ * workgroups).  Calls on one device share that workspace: calls that
+ * overlap on one GPU (different streams, no ordering) race; calls ordered
+ * on one stream do not.  Results are bitwise those of scxr_vec_reduce. */
+int scxr_vec_reduce_noalloc(int32_t kind, int64_t n, int32_t K, const float* x,
+                            const float* y, double scale, int32_t post, double* out,
+                            const int32_t* pred, void* stream);
 
 /* ---------------------------------------------- finite differences, prox --
  * FiniteDifference (scico/linop/_diff.py:25-272) over all axes of an

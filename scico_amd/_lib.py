@@ -167,6 +167,9 @@ SIGNATURES = [
     ("scxr_vec_reduce", c_int32,
      [c_int32, c_int64, c_int32, c_void_p, c_void_p, c_double, c_int32, c_void_p, c_void_p,
       c_void_p, c_void_p]),
+    ("scxr_vec_reduce_noalloc", c_int32,
+     [c_int32, c_int64, c_int32, c_void_p, c_void_p, c_double, c_int32, c_void_p, c_void_p,
+      c_void_p]),
 ]
 
 _lib = None

@@ -8,6 +8,9 @@
 #include "common.h"
 #include "vec_impl.h"
 
+#include <atomic>
+#include <mutex>
+
 using namespace scxr;
 
 namespace scxr {
@@ -163,6 +166,41 @@ int sum_partials(int nparts, const double* parts, double scale, int post, double
   return SCXR_OK;
 }
 
+// Workspace of scxr_vec_reduce_noalloc: the first pass's float64 partials
+// (one per workgroup, at most RED_BLOCKS), 8 KiB held by every device the
+// library runs on until the process ends.  Host code never passes this
+// name to a kernel: the name is a host-side stand-in, the device address
+// comes from hipGetSymbolAddress, once per device.
+constexpr int NOALLOC_PARTIALS = RED_BLOCKS;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static std::atomic<double*> g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of the current device's copy of `scratch`.
+static int scratch_of_current_device(double** out) {
+  int dev = 0;
+  SCXR_HIP(hipGetDevice(&dev));
+  SCXR_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, SCXR_ERR_UNSUPPORTED,
+               "device ordinal %d is beyond the %d devices the static workspace supports", dev,
+               NOALLOC_MAX_DEVICES);
+  double* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void* addr = nullptr;
+      SCXR_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+      SCXR_REQUIRE(addr != nullptr, SCXR_ERR_HIP, "no device address for the static workspace");
+      p = static_cast<double*>(addr);
+      g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return SCXR_OK;
+}
+
 }  // namespace vec
 }  // namespace scxr
 
@@ -254,6 +292,30 @@ int scxr_vec_reduce(int32_t kind, int64_t n, int32_t K, const float* x, const fl
   SCXR_REQUIRE((kind != vec::RED_DOT && kind != vec::RED_SQDIST) || y, SCXR_ERR_ARG,
                "reduction needs a second operand");
   SCXR_REQUIRE(kind != vec::RED_L21 || K >= 1, SCXR_ERR_ARG, "L21 reduction needs K >= 1");
+  return vec::reduce(kind, n, K, x, y, scale, post, out, ws, pred, as_stream(stream));
+}
+
+int scxr_vec_reduce_noalloc(int32_t kind, int64_t n, int32_t K, const float* x, const float* y,
+                            double scale, int32_t post, double* out, const int32_t* pred,
+                            void* stream) {
+  SCXR_REQUIRE(kind >= vec::RED_DOT && kind <= vec::RED_L21, SCXR_ERR_ARG, "bad reduction kind");
+  SCXR_REQUIRE(n >= 0 && x && out, SCXR_ERR_ARG, "bad reduction arguments");
+  SCXR_REQUIRE((kind != vec::RED_DOT && kind != vec::RED_SQDIST) || y, SCXR_ERR_ARG,
+               "reduction needs a second operand");
+  SCXR_REQUIRE(kind != vec::RED_L21 || K >= 1, SCXR_ERR_ARG, "L21 reduction needs K >= 1");
+  // The first pass's grid, hence the partial count, is that of vec::reduce:
+  // one workgroup per 1024 elements, at least 1 and at most RED_BLOCKS.
+  static_assert(vec::NOALLOC_PARTIALS >= vec::RED_BLOCKS, "workspace below the grid cap");
+  const int64_t blocks = (n + 1023) / 1024;
+  const int64_t parts = blocks < 1 ? 1 : (blocks > vec::RED_BLOCKS ? vec::RED_BLOCKS : blocks);
+  SCXR_REQUIRE(parts <= vec::NOALLOC_PARTIALS, SCXR_ERR_ARG,
+               "reduction needs %lld partials, the static workspace holds %d",
+               static_cast<long long>(parts), vec::NOALLOC_PARTIALS);
+  double* ws = nullptr;
+  const int rc = vec::scratch_of_current_device(&ws);
+  if (rc) return rc;
+  // Every partial the second pass reads is written by this call's first
+  // pass, so nothing left by an earlier call needs clearing.
   return vec::reduce(kind, n, K, x, y, scale, post, out, ws, pred, as_stream(stream));
 }
 
