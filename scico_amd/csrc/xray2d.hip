@@ -98,7 +98,8 @@ static int lane_mult_for(double d) {
 }
 
 int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
-                 double y0, const float* angles, int vpg, int ppt) {
+                 double y0, const float* angles, int vpg, int ppt, int fwd_ch,
+                 bool allow_generic) {
   view_geometry(A, angles, x0, dx, y0, h.geom, h.rw);
   const std::vector<float4>& geom = h.geom;
   for (int a = 0; a < A; ++a)
@@ -115,11 +116,16 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
   for (int a = 0; a < A; ++a)
     (std::fabs(geom[a].z) >= std::fabs(geom[a].y) ? rows : cols).push_back(a);
   h.any_colfast = !cols.empty();
-  SCXR_REQUIRE(fwd_chains(vpg, D) > 0, SCXR_ERR_UNSUPPORTED,
-               "det_count %d too large for the LDS-resident forward projector", D);
+  if (fwd_ch < 0) fwd_ch = fwd_chains(vpg, D);
+  h.fwd_generic = false;
+  if (fwd_ch == 0) {
+    SCXR_REQUIRE(allow_generic, SCXR_ERR_UNSUPPORTED,
+                 "det_count %d too large for the LDS-resident forward projector", D);
+    h.fwd_generic = true;
+  }
   h.order.clear();
   h.groups.clear();
-  for (int cls = 0; cls < 2; ++cls) {
+  for (int cls = 0; cls < 2 && !h.fwd_generic; ++cls) {
     const std::vector<int>& list = cls ? cols : rows;
     for (size_t s = 0; s < list.size(); s += vpg) {
       const int cnt = static_cast<int>(std::min<size_t>(vpg, list.size() - s));
@@ -131,9 +137,13 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
         const double pf = std::max(std::fabs(static_cast<double>(cls ? g.y : g.z)), 1e-30);
         S = std::max(S, static_cast<int>(std::ceil(2.0625 / pf)));
       }
-      SCXR_REQUIRE(S <= FWD_SMAX, SCXR_ERR_GEOM,
-                   "projected pixel spacing too small for the forward projector "
-                   "(lane stride %d > %d)", S, FWD_SMAX);
+      if (S > FWD_SMAX) {
+        SCXR_REQUIRE(allow_generic, SCXR_ERR_GEOM,
+                     "projected pixel spacing too small for the forward projector "
+                     "(lane stride %d > %d)", S, FWD_SMAX);
+        h.fwd_generic = true;
+        break;
+      }
       // A larger stride keeps the footprints disjoint too; among the
       // compiled strides take the one with the fewest phases per line,
       // ceil(Lf / 64 S) * S (e.g. 4 rather than 3 for 512-pixel lines: 8
@@ -195,12 +205,23 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
     windows(ppt, h.clo, h.adj_w);
     h.adj_ppt = ppt;
   }
-  SCXR_REQUIRE(h.adj_w <= ADJ_WFAST, SCXR_ERR_GEOM,
-               "projected pixel footprint too large for the adjoint tile (%d bins > %d); "
-               "pixel size dx is too large relative to the detector spacing",
-               h.adj_w, ADJ_WFAST);
+  h.adj_generic = false;
+  if (h.adj_w > ADJ_WFAST) {
+    SCXR_REQUIRE(allow_generic, SCXR_ERR_GEOM,
+                 "projected pixel footprint too large for the adjoint tile (%d bins > %d); "
+                 "pixel size dx is too large relative to the detector spacing",
+                 h.adj_w, ADJ_WFAST);
+    h.adj_generic = true;
+  }
+  if (h.fwd_generic) {
+    h.groups.clear();
+    h.order.assign(A, 0);
+    for (int a = 0; a < A; ++a) h.order[a] = a;
+  }
   return SCXR_OK;
 }
+
+static hipError_t fwd_allow_lds(int bytes);
 
 int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx,
            double y0, const float* angles) {
@@ -211,10 +232,29 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   // Views per workgroup: enough workgroups to fill 256 CUs at least twice.
   p.vpg = A >= 2048 ? 4 : (A >= 1024 ? 2 : 1);
   while (p.vpg > 1 && fwd_chains(p.vpg, D) == 0) p.vpg /= 2;
-  p.fwd_ch = fwd_chains(p.vpg, D);
+  size_t lds_cap = FWD_LDS_MAX;
+  {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      lds_cap = std::max(lds_cap, static_cast<size_t>(v));
+  }
+  p.fwd_ch = fwd_chains_ext(p.vpg, D, lds_cap);
+  if (p.fwd_ch > 0 && fwd_lds_bytes(p.vpg, D, p.fwd_ch) > FWD_LDS_MAX) {
+    // rows beyond 64 KiB: raise the kernel's dynamic LDS limit, or take the
+    // general-geometry forward if the runtime refuses
+    const int bytes = static_cast<int>(fwd_lds_bytes(p.vpg, D, p.fwd_ch));
+    if (fwd_allow_lds(bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      p.fwd_ch = 0;
+    }
+  }
   HostTables h;
-  const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg, 0);
+  const int rc = build_tables(h, M, N, A, D, x0, dx, y0, angles, p.vpg, 0, p.fwd_ch, true);
   if (rc) return rc;
+  p.fwd_generic = h.fwd_generic;
+  p.adj_generic = h.adj_generic;
   p.adj_ppt = h.adj_ppt;
   p.adj_fract = h.px_ge1;
   p.host_geom = h.geom;
@@ -343,7 +383,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 // views t = Pxmin + P0 * i is constant along the line, for column-fast
 // views t = P1 * j is.
 // Accumulate one phase (lanes at f = c0 + S lane + r) of the CH lines.
-template <int V, int CH, bool COLFAST>
+template <int V, int CH, bool COLFAST, bool FR>
 __device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
                                           const float (&t)[CH][V], const float* val,
                                           float ff, int cnt, const float* Pxmin,
@@ -359,9 +399,14 @@ __device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const float P = COLFAST ? __fadd_rn(tl, t[c][v]) : __fadd_rn(t[c][v], tl);
-        const float fl = floorf(P);
-        w[c] = div_rn_sat(__fsub_rn(1.0f, __fsub_rn(P, fl)), rw[v]);
-        k[c] = row[c][v] + static_cast<int>(__builtin_amdgcn_fmed3f(fl, -2.0f, Dmax));
+        if (FR) {  // every Px >= 1: the fraction instruction (see weight_frac)
+          w[c] = weight_frac(P, rw[v]);
+          k[c] = row[c][v] + cvt_flr_i32(fminf(P, Dmax));
+        } else {
+          const float fl = floorf(P);
+          w[c] = div_rn_sat(__fsub_rn(1.0f, __fsub_rn(P, fl)), rw[v]);
+          k[c] = row[c][v] + static_cast<int>(__builtin_amdgcn_fmed3f(fl, -2.0f, Dmax));
+        }
       }
       float a0[CH], a1[CH];
 #pragma unroll
@@ -388,7 +433,7 @@ __device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
 // atomics.  The walk over (line group, block c0, phase r) is flat and the
 // image values of the next phase are loaded before the current phase is
 // accumulated.
-template <int V, int CH, bool COLFAST>
+template <int V, int CH, bool COLFAST, bool FR>
 __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* acc, int L,
                                          int cnt, int S, int kmul, const float* Pxmin,
                                          const float* P0, const float* P1, const double* rw,
@@ -462,7 +507,7 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
       }
     }
     const float ff = static_cast<float>(c0 + loff + r);
-    fwd_phase<V, CH, COLFAST>(acc, row, t, val, ff, cnt, Pxmin, P0, P1, rw, Dmax);
+    fwd_phase<V, CH, COLFAST, FR>(acc, row, t, val, ff, cnt, Pxmin, P0, P1, rw, Dmax);
     if (sn >= Ls) break;
   }
 }
@@ -473,7 +518,7 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
 // ahead into one of two register buffers that alternate between blocks,
 // so the per-phase loop bookkeeping and register copies of the runtime-S
 // walk disappear.
-template <int V, int CH, bool COLFAST, int S>
+template <int V, int CH, bool COLFAST, int S, bool FR>
 __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float* acc, int L,
                                            int cnt, int kmul, const float* Pxmin, const float* P0,
                                            const float* P1, const double* rw, int M, int N,
@@ -563,7 +608,7 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
       float val[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) val[c] = cur[c][r];
-      fwd_phase<V, CH, COLFAST>(acc, row, t, val, static_cast<float>(c0 + loff + r), cnt, Pxmin,
+      fwd_phase<V, CH, COLFAST, FR>(acc, row, t, val, static_cast<float>(c0 + loff + r), cnt, Pxmin,
                                 P0, P1, rw, Dmax);
     }
     if (more && sn != s0) set_t(sn);
@@ -578,7 +623,7 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
 }
 
 // One forward group (gi indexes groups[]); acc is the dynamic LDS array.
-template <int V, int CH>
+template <int V, int CH, bool FR>
 __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
                                           const float* __restrict__ imT,
                                           float* __restrict__ sino,
@@ -615,9 +660,9 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   const float* src = colf ? imT : im;
 #define SCXR_FWD_S(SS)                                                                 \
   if (colf)                                                                            \
-    fwd_body_s<V, CH, true, SS>(src, acc, L, cnt, kmul, Pxmin, P0, P1, rw, M, N, D);   \
+    fwd_body_s<V, CH, true, SS, FR>(src, acc, L, cnt, kmul, Pxmin, P0, P1, rw, M, N, D);   \
   else                                                                                 \
-    fwd_body_s<V, CH, false, SS>(src, acc, L, cnt, kmul, Pxmin, P0, P1, rw, M, N, D);
+    fwd_body_s<V, CH, false, SS, FR>(src, acc, L, cnt, kmul, Pxmin, P0, P1, rw, M, N, D);
   switch (grp.w) {
     case 3:
       SCXR_FWD_S(3)
@@ -630,9 +675,9 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
       break;
     default:
       if (colf)
-        fwd_body<V, CH, true>(src, acc, L, cnt, grp.w, kmul, Pxmin, P0, P1, rw, M, N, D);
+        fwd_body<V, CH, true, FR>(src, acc, L, cnt, grp.w, kmul, Pxmin, P0, P1, rw, M, N, D);
       else
-        fwd_body<V, CH, false>(src, acc, L, cnt, grp.w, kmul, Pxmin, P0, P1, rw, M, N, D);
+        fwd_body<V, CH, false, FR>(src, acc, L, cnt, grp.w, kmul, Pxmin, P0, P1, rw, M, N, D);
   }
 #undef SCXR_FWD_S
   __syncthreads();
@@ -668,7 +713,7 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   }
 }
 
-template <int V, int CH>
+template <int V, int CH, bool FR>
 __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_kernel(const float* __restrict__ im,
                                                   const float* __restrict__ imT,
                                                   float* __restrict__ sino,
@@ -681,8 +726,18 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_kernel(const float* __rest
                                                   const int32_t* pred) {
   if (pred_off(pred)) return;
   extern __shared__ float acc[];  // [V][FWD_WAVES waves][CH chains][L]
-  fwd_group<V, CH>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws, blockIdx.x,
-                   acc);
+  fwd_group<V, CH, FR>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws,
+                       blockIdx.x, acc);
+}
+
+// Dynamic LDS beyond 64 KiB for the one-view, one-chain forward (wide
+// detectors).
+static hipError_t fwd_allow_lds(int bytes) {
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fwd_kernel<1, 1, false>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&fwd_kernel<1, 1, true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 // Adjoint (back projection).  One workgroup per pixel tile of ADJ_TI rows
@@ -945,6 +1000,81 @@ __global__ __launch_bounds__(256) void ramp_kernel(const float* __restrict__ y,
   }
 }
 
+// ------------------------------------------------ general geometries --
+// Plans whose geometry the tiled kernels cannot hold (a detector row wider
+// than the LDS, projected pixel spacing below 1/32 bin, or a pixel tile
+// projecting over more than 64 bins) run these two gathers instead: the
+// same bins and weights (weight_at), deterministic, no LDS staging.
+
+// Adjoint: one thread per pixel, views in order, the two sums of
+// _xray2d.py:299-304 kept apart and added at the end.
+__global__ __launch_bounds__(256) void adj_generic_kernel(const float* __restrict__ sino,
+                                                          float* __restrict__ im,
+                                                          const float4* __restrict__ geom,
+                                                          const double* __restrict__ rwt, int M,
+                                                          int N, int A, int D,
+                                                          const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const long long e = blockIdx.x * 256ll + threadIdx.x;
+  if (e >= static_cast<long long>(M) * N) return;
+  const int i = static_cast<int>(e / N), j = static_cast<int>(e - static_cast<long long>(i) * N);
+  const float fi = static_cast<float>(i), fj = static_cast<float>(j);
+  const float Dmax = static_cast<float>(D);
+  float s0 = 0.0f, s1 = 0.0f;
+  for (int a = 0; a < A; ++a) {
+    const float4 g = geom[a];
+    const Weight wt = weight_at(g.x, g.y, g.z, g.w, rwt[a], fi, fj);
+    const int k = static_cast<int>(__builtin_amdgcn_fmed3f(wt.fl, -2.0f, Dmax));
+    const float* y = sino + static_cast<size_t>(a) * D;
+    if (k >= 0 && k < D) s0 = fmaf(y[k], wt.w, s0);
+    if (k + 1 >= 0 && k + 1 < D) s1 = fmaf(y[k + 1], __fsub_rn(1.0f, wt.w), s1);
+  }
+  im[e] = __fadd_rn(s0, s1);
+}
+
+// Forward: one thread per (bin b, view a).  Along the slow image axis s the
+// projected coordinate is monotone in the fast index f, so the pixels of a
+// line whose bin is b or b - 1 lie in a short run of f; the run is located
+// in float64 (widened by one pixel each side) and every candidate's bin is
+// then decided by the exact float32 arithmetic.
+__global__ __launch_bounds__(256) void fwd_generic_kernel(const float* __restrict__ im,
+                                                          float* __restrict__ sino,
+                                                          const float4* __restrict__ geom,
+                                                          const double* __restrict__ rwt, int M,
+                                                          int N, int D, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  const int a = blockIdx.y;
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= D) return;
+  const float4 g = geom[a];
+  const double rw = rwt[a];
+  const bool rowfast = fabsf(g.z) >= fabsf(g.y);
+  const int Ls = rowfast ? M : N, Lf = rowfast ? N : M;
+  const double Pf = rowfast ? g.z : g.y, Ps = rowfast ? g.y : g.z;
+  const float Dmax = static_cast<float>(D);
+  // bound of the float32 rounding of Px (three roundings of terms at most
+  // |Pxmin| + |Ps| Ls + |Pf| Lf), added to the bin interval [b - 1, b + 1)
+  const double del = 4e-7 * (fabs(static_cast<double>(g.x)) + fabs(Ps) * Ls + fabs(Pf) * Lf) + 1e-6;
+  float acc = 0.0f;
+  for (int s = 0; s < Ls; ++s) {
+    const double c = static_cast<double>(g.x) + Ps * s;
+    const double f0 = (b - 1 - del - c) / Pf, f1 = (b + 1 + del - c) / Pf;
+    const int lo = max(0, static_cast<int>(floor(fmin(f0, f1))) - 1);
+    const int hi = min(Lf - 1, static_cast<int>(ceil(fmax(f0, f1))) + 1);
+    for (int f = lo; f <= hi; ++f) {
+      const int i = rowfast ? s : f, j = rowfast ? f : s;
+      const Weight wt = weight_at(g.x, g.y, g.z, g.w, rw, static_cast<float>(i),
+                                  static_cast<float>(j));
+      const int k = static_cast<int>(__builtin_amdgcn_fmed3f(wt.fl, -2.0f, Dmax));
+      if (k == b || k == b - 1) {
+        const float v = im[static_cast<size_t>(i) * N + j];
+        acc = fmaf(v, k == b ? wt.w : __fsub_rn(1.0f, wt.w), acc);
+      }
+    }
+  }
+  sino[static_cast<size_t>(a) * D + b] = acc;
+}
+
 // Debug dump of (ind, weight) for every (view, pixel).
 __global__ __launch_bounds__(256) void weights_kernel(int32_t* __restrict__ inds,
                                                       float* __restrict__ wts,
@@ -995,7 +1125,7 @@ __global__ __launch_bounds__(256) void transpose_batch_kernel(BatchTabs t,
   }
 }
 
-template <int V, int CH>
+template <int V, int CH, bool FR>
 __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_batch_kernel(BatchTabs t, const float* __restrict__ im,
                                                         const float* __restrict__ imT,
                                                         float* __restrict__ sino,
@@ -1010,7 +1140,7 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_batch_kernel(BatchTabs t, 
     const int4 sh = t.shape[s];
     const int64_t io = t.ioff[s], so = t.soff[s];
     const int vo = t.voff[s], go = t.goff[s];
-    fwd_group<V, CH>(im + io, imT + io, sino + so, t.geom + vo, t.rw + vo, t.groups + go,
+    fwd_group<V, CH, FR>(im + io, imT + io, sino + so, t.geom + vo, t.rw + vo, t.groups + go,
                  t.order + vo, sh.x, sh.y, sh.w, yres ? yres + so : nullptr,
                  res_ws ? res_ws + go : nullptr, w.y, acc);
     __syncthreads();  // the next item re-zeroes acc
@@ -1054,9 +1184,14 @@ int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im
   const unsigned g = grid > 0 ? std::min(grid, t.nf) : t.nf;
   const size_t lds = fwd_lds_bytes(V, Dmax, t.ch);
 #define SCXR_FWDB(VV, CC)                                                                    \
-  hipLaunchKernelGGL((fwd_batch_kernel<VV, CC>), dim3(g), dim3(64 * FWD_WAVES), lds, s, t, im, \
-                     imT, sino, \
-                     yres, res_ws, pred, pstride)
+  do {                                                                                       \
+    if (t.fract)                                                                             \
+      hipLaunchKernelGGL((fwd_batch_kernel<VV, CC, true>), dim3(g), dim3(64 * FWD_WAVES), lds, s, \
+                         t, im, imT, sino, yres, res_ws, pred, pstride);                     \
+    else                                                                                     \
+      hipLaunchKernelGGL((fwd_batch_kernel<VV, CC, false>), dim3(g), dim3(64 * FWD_WAVES), lds, \
+                         s, t, im, imT, sino, yres, res_ws, pred, pstride);                  \
+  } while (0)
   if (t.ch == 2) {
     if (V == 1)
       SCXR_FWDB(1, 2);
@@ -1101,6 +1236,15 @@ int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
 
 int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,
             const float* yres, double* res_ws) {
+  if (p.fwd_generic) {
+    SCXR_REQUIRE(!res_ws, SCXR_ERR_UNSUPPORTED,
+                 "residual partials are not available for this geometry (general-geometry "
+                 "forward)");
+    hipLaunchKernelGGL(fwd_generic_kernel, dim3(ceil_div(p.D, 256), p.A), dim3(256), 0, s, im,
+                       sino, p.geom, p.rw, p.M, p.N, p.D, pred);
+    SCXR_LAUNCHED("xr2d general forward");
+    return SCXR_OK;
+  }
   if (p.any_colfast) {
     dim3 g(ceil_div(p.N, 32), ceil_div(p.M, 32));
     hipLaunchKernelGGL(transpose_kernel, g, dim3(256), 0, s, im, p.work, p.M, p.N, pred);
@@ -1108,9 +1252,16 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
   }
   const size_t lds = fwd_lds_bytes(p.vpg, p.D, p.fwd_ch);
 #define SCXR_FWD(VV, CC)                                                                      \
-  hipLaunchKernelGGL((fwd_kernel<VV, CC>), dim3(p.n_groups), dim3(64 * FWD_WAVES), lds, s, im, \
-                     p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres, res_ws, \
-                     pred)
+  do {                                                                                         \
+    if (p.adj_fract)                                                                           \
+      hipLaunchKernelGGL((fwd_kernel<VV, CC, true>), dim3(p.n_groups), dim3(64 * FWD_WAVES), lds, \
+                         s, im, p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, \
+                         yres, res_ws, pred);                                                \
+    else                                                                                       \
+      hipLaunchKernelGGL((fwd_kernel<VV, CC, false>), dim3(p.n_groups), dim3(64 * FWD_WAVES),   \
+                         lds, s, im, p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, \
+                         p.D, yres, res_ws, pred);                                           \
+  } while (0)
   if (p.fwd_ch == 2) {
     if (p.vpg == 1)
       SCXR_FWD(1, 2);
@@ -1133,6 +1284,16 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
 
 int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,
             const AdjEpi& epi) {
+  if (p.adj_generic) {
+    SCXR_REQUIRE(epi.kind == EPI_NONE && !epi.dot_ws, SCXR_ERR_UNSUPPORTED,
+                 "adjoint epilogues are not available for this geometry (general-geometry "
+                 "adjoint)");
+    const long long n = static_cast<long long>(p.M) * p.N;
+    hipLaunchKernelGGL(adj_generic_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, sino, im,
+                       p.geom, p.rw, p.M, p.N, p.A, p.D, pred);
+    SCXR_LAUNCHED("xr2d general adjoint");
+    return SCXR_OK;
+  }
   dim3 g(ceil_div(p.N, ADJ_TJ * p.adj_ppt), ceil_div(p.M, ADJ_TI));
 #define SCXR_ADJ(PP, FF)                                                                    \
   hipLaunchKernelGGL((adj_kernel<PP, FF>), g, dim3(256 * ADJ_VS), 0, s, sino, im, p.geom, p.rw,  \
@@ -1241,8 +1402,10 @@ int scxr_xr2d_adjoint_epi(const scxr_xr2d_plan* plan, const float* sino, float* 
 int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out) {
   SCXR_REQUIRE(plan && out, SCXR_ERR_ARG, "null argument to xr2d_info");
   const auto& p = plan->p;
-  out[0] = p.n_groups;
-  out[1] = static_cast<int32_t>(ceil_div(p.N, ADJ_TJ * p.adj_ppt) * ceil_div(p.M, ADJ_TI));
+  out[0] = p.fwd_generic ? 0 : p.n_groups;
+  out[1] = p.adj_generic
+               ? 0
+               : static_cast<int32_t>(ceil_div(p.N, ADJ_TJ * p.adj_ppt) * ceil_div(p.M, ADJ_TI));
   out[2] = p.vpg;
   out[3] = p.adj_w;
   return SCXR_OK;

@@ -34,6 +34,13 @@ inline int fwd_chains(int vpg, int D) {
   if (fwd_lds_bytes(vpg, D, 1) <= FWD_LDS_MAX) return 1;
   return 0;
 }
+// Single-slice plans of one view per workgroup may use the device's whole
+// LDS (160 KiB on gfx950) for detectors too wide for FWD_LDS_MAX.
+inline int fwd_chains_ext(int vpg, int D, size_t lds_cap) {
+  const int c = fwd_chains(vpg, D);
+  if (c > 0) return c;
+  return (vpg == 1 && fwd_lds_bytes(1, D, 1) <= lds_cap) ? 1 : 0;
+}
 
 struct Plan {
   int M = 0, N = 0, A = 0, D = 0;
@@ -44,6 +51,9 @@ struct Plan {
   int adj_ppt = 1;     // adjoint pixels per thread
   bool adj_fract = false;  // Px >= 1 everywhere: fraction-instruction weights
   bool any_colfast = false;
+  // general-geometry gathers (fwd_generic_kernel / adj_generic_kernel) in
+  // place of the tiled kernels, which cannot hold this geometry
+  bool fwd_generic = false, adj_generic = false;
   void* dev_block = nullptr;  // device tables (one allocation)
   float4* geom = nullptr;     // (Pxmin, P0, P1, width) per view
   double* rw = nullptr;       // RN64(1 / width) per view
@@ -80,11 +90,15 @@ struct HostTables {
   int adj_ppt = 1;
   bool any_colfast = false;
   bool px_ge1 = false;  // Px >= 1 for every pixel and view
+  bool fwd_generic = false, adj_generic = false;  // see Plan
 };
 // vpg: views per forward group; ppt: adjoint pixels per thread (0 = 2 when
-// the adjoint window fits, else 1).
+// the adjoint window fits, else 1).  fwd_ch: forward chains per wave (0:
+// the rows do not fit).  With allow_generic, geometries the tiled kernels
+// cannot hold set fwd_generic / adj_generic instead of failing.
 int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
-                 double y0, const float* angles, int vpg, int ppt);
+                 double y0, const float* angles, int vpg, int ppt, int fwd_ch = -1,
+                 bool allow_generic = false);
 
 // Batched (ragged) slices: device tables of a batch plan.  Slice s has
 // shape (M, N, A, D) = shape[s]; its image starts at ioff[s] in packed

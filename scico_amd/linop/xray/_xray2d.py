@@ -118,6 +118,22 @@ class XRayTransform2D(LinearOperator):
             self._finalizer = weakref.finalize(self, lib.scxr_xr2d_plan_destroy, h)
         return self._plan
 
+    def launch_info(self) -> dict:
+        """Launch facts of the plan (``scxr_xr2d_info``): forward and adjoint
+        workgroups (0 when that direction runs the general-geometry gather,
+        which has no fused epilogues), views per forward workgroup, adjoint
+        window width."""
+        out = (ctypes.c_int32 * 4)()
+        call("scxr_xr2d_info", self.plan, out)
+        return {"forward_groups": int(out[0]), "adjoint_tiles": int(out[1]),
+                "views_per_group": int(out[2]), "adjoint_window": int(out[3])}
+
+    @property
+    def tiled(self) -> bool:
+        """Both directions run the tiled kernels (with fused epilogues)."""
+        i = self.launch_info()
+        return i["forward_groups"] > 0 and i["adjoint_tiles"] > 0
+
     def view_table(self) -> np.ndarray:
         """Per-view float32 (Pxmin, P0, P1, width) as computed by the plan."""
         out = np.empty((self.output_shape[0], 4), dtype=np.float32)

@@ -201,8 +201,9 @@ class _FusedTV:
         self.iso = bool(isotropic)
         self.rho = float(admm.rho_list[0])
         self.circ = int(self.C.circular)
-        # 2D projector: FD terms fused into the projector epilogues
-        self.epi = isinstance(self.A, XRayTransform2D)
+        # 2D projector: FD terms fused into the projector epilogues (the
+        # general-geometry kernels have none: nd passes instead)
+        self.epi = isinstance(self.A, XRayTransform2D) and self.A.tiled
         # forward / back projection with out= and pred= (for the identity
         # of a denoising loss, a predicated copy)
         if type(self.A) is Identity:
