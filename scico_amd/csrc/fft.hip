@@ -17,6 +17,8 @@
 // the last one writes the real part.  With d == 1 the single fused pass
 // does both.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "common.h"
@@ -437,6 +439,114 @@ __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __rest
   }
 }
 
+// ------------------------------------------- real 2D convolution path --
+//
+// For two power-of-two FFT axes (F0, F1) the convolution of real data runs
+// on half spectra.  Rows are transformed two at a time: z = x_a + i x_b
+// has the DFT Z = X_a + i X_b with X_a, X_b Hermitian, so
+//   X_a[k] = (Z[k] + conj(Z[-k])) / 2,   X_b[k] = (Z[k] - conj(Z[-k])) / (2 i),
+// and only k = 0 .. F1/2 of each row is kept (H1 = F1/2 + 1 columns).  The
+// columns of that half spectrum take the fused pass (forward, times the
+// spectrum, inverse).  real(ifft2(H X)) = irfft2(Hs X) with the Hermitian
+// part Hs[k] = (H[k] + conj(H[-k])) / 2 of H (equal to H for a real
+// kernel), so the plan keeps Hs on the half grid.  The inverse rows rebuild
+// Z = X_a + i X_b on the full row from the Hermitian extensions (the
+// imaginary parts of the k = 0 and k = F1/2 bins dropped, as irfft does)
+// and write x_a = Re z, x_b = Im z.
+
+// Forward rows: real x (R rows of n) -> half spectra (R rows of n/2 + 1).
+__global__ __launch_bounds__(SH_THREADS) void r2c_rows_kernel(const float* __restrict__ x,
+                                                              float2* __restrict__ half, int n,
+                                                              int log2n, long long R, int lpb,
+                                                              const float2* __restrict__ tw,
+                                                              const int32_t* pred) {
+  if (pred_off(pred)) return;
+  extern __shared__ float2 lds[];  // lpb * n values
+  float2* X = lds;
+  const int tot = lpb * n, H1 = n / 2 + 1;
+  const long long row0 = 2ll * blockIdx.x * lpb;  // first real row of the workgroup
+  {
+    float2 vv[SH_MAXV];
+#pragma unroll
+    for (int q = 0; q < SH_MAXV; ++q) {
+      const int e = threadIdx.x + q * SH_THREADS;
+      if (e < tot) {
+        const int l = e / n, k = e - l * n;
+        const long long ra = row0 + 2 * l, rb = ra + 1;
+        const float a = ra < R ? x[ra * n + k] : 0.0f;
+        const float b = rb < R ? x[rb * n + k] : 0.0f;
+        vv[q] = make_float2(a, b);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SH_MAXV; ++q) {
+      const int e = threadIdx.x + q * SH_THREADS;
+      if (e < tot) X[e] = vv[q];
+    }
+  }
+  __syncthreads();
+  stockham(X, n, log2n, lpb, tw, false);
+  for (int e = threadIdx.x; e < lpb * H1; e += SH_THREADS) {
+    const int l = e / H1, k = e - l * H1;
+    const long long ra = row0 + 2 * l, rb = ra + 1;
+    const float2 zk = X[l * n + k], zm = X[l * n + ((n - k) & (n - 1))];
+    if (ra < R)
+      half[ra * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.x, zm.x)),
+                                      __fmul_rn(0.5f, __fsub_rn(zk.y, zm.y)));
+    if (rb < R)
+      half[rb * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.y, zm.y)),
+                                      __fmul_rn(0.5f, __fsub_rn(zm.x, zk.x)));
+  }
+}
+
+// Inverse rows: half spectra (R rows of n/2 + 1) -> real y (R rows of n),
+// times scale.
+__global__ __launch_bounds__(SH_THREADS) void c2r_rows_kernel(const float2* __restrict__ half,
+                                                              float* __restrict__ y, int n,
+                                                              int log2n, long long R, int lpb,
+                                                              const float2* __restrict__ tw,
+                                                              float scale, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  extern __shared__ float2 lds[];
+  float2* X = lds;
+  const int H1 = n / 2 + 1, nh = n / 2;
+  const long long row0 = 2ll * blockIdx.x * lpb;
+  for (int e = threadIdx.x; e < lpb * H1; e += SH_THREADS) {
+    const int l = e / H1, k = e - l * H1;
+    const long long ra = row0 + 2 * l, rb = ra + 1;
+    float2 a = ra < R ? half[ra * H1 + k] : make_float2(0.0f, 0.0f);
+    float2 b = rb < R ? half[rb * H1 + k] : make_float2(0.0f, 0.0f);
+    if (k == 0 || k == nh) a.y = b.y = 0.0f;
+    // Z[k] = A[k] + i B[k];  Z[n - k] = conj(A[k]) + i conj(B[k])
+    X[l * n + k] = make_float2(__fsub_rn(a.x, b.y), __fadd_rn(a.y, b.x));
+    if (k > 0 && k < nh) X[l * n + n - k] = make_float2(__fadd_rn(a.x, b.y), __fsub_rn(b.x, a.y));
+  }
+  __syncthreads();
+  stockham(X, n, log2n, lpb, tw, true);
+  for (int e = threadIdx.x; e < lpb * n; e += SH_THREADS) {
+    const int l = e / n, k = e - l * n;
+    const long long ra = row0 + 2 * l, rb = ra + 1;
+    const float2 v = X[e];
+    if (ra < R) y[ra * n + k] = __fmul_rn(v.x, scale);
+    if (rb < R) y[rb * n + k] = __fmul_rn(v.y, scale);
+  }
+}
+
+// Hermitian part of the (F0, F1) spectrum on the half grid (F0, F1/2 + 1).
+__global__ __launch_bounds__(256) void hermitian_half_kernel(const float2* __restrict__ h,
+                                                             float2* __restrict__ hs, int F0,
+                                                             int F1) {
+  const int H1 = F1 / 2 + 1;
+  const long long tot = static_cast<long long>(F0) * H1;
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < tot; e += 256ll * gridDim.x) {
+    const int k0 = static_cast<int>(e / H1), k1 = static_cast<int>(e - static_cast<long long>(k0) * H1);
+    const float2 a = h[static_cast<long long>(k0) * F1 + k1];
+    const int m0 = (F0 - k0) % F0, m1 = (F1 - k1) % F1;
+    const float2 b = h[static_cast<long long>(m0) * F1 + m1];
+    hs[e] = make_float2(__fmul_rn(0.5f, __fadd_rn(a.x, b.x)), __fmul_rn(0.5f, __fsub_rn(a.y, b.y)));
+  }
+}
+
 // Elementwise complex product c = a * b (spectrum centring shift).
 __global__ __launch_bounds__(256) void cmul_kernel(long long n, const float2* __restrict__ a,
                                                    const float2* __restrict__ b,
@@ -460,6 +570,12 @@ struct Plan {
   AxisTables ax[3];
   float2* h = nullptr;     // spectrum (fsize)
   float2* work = nullptr;  // complex workspace (batch * fsize)
+  // real 2D path (two power-of-two axes, F1 >= 4): Hermitian part of the
+  // spectrum on the half grid (F0, F1/2 + 1) and lines per workgroup of the
+  // row and column passes
+  bool r2c = false;
+  float2* hs = nullptr;
+  int lpb_rows = 1, lpb_cols = 1;
 };
 
 int log2_exact(int n) {
@@ -490,6 +606,7 @@ int make_tables(AxisTables& t, int n) {
 void destroy(Plan& p) {
   for (auto& a : p.ax)
     if (a.tw) (void)hipFree(a.tw), a.tw = nullptr;
+  if (p.hs) (void)hipFree(p.hs), p.hs = nullptr;
   if (p.h) (void)hipFree(p.h), p.h = nullptr;
   if (p.work) (void)hipFree(p.work), p.work = nullptr;
 }
@@ -559,9 +676,48 @@ int run_pass(const Plan& p, int a, const void* in, void* out, bool in_real, bool
   return SCXR_OK;
 }
 
+// Real 2D path: rows forward (two real rows per complex FFT), fused column
+// pass on the half spectrum, rows inverse.
+int convolve_r2c(const Plan& p, const float* x, float* y, bool adj, const int32_t* pred,
+                 hipStream_t s) {
+  const AxisTables& t1 = p.ax[1];
+  const int n = t1.n, H1 = n / 2 + 1;
+  const long long R = p.batch * p.F[0];
+  float2* half = p.work;  // R rows of H1 (fits in the batch * fsize workspace)
+  const long long cl = (R + 1) / 2;  // complex row transforms
+  int lr = p.lpb_rows;
+  const unsigned gr = static_cast<unsigned>((cl + lr - 1) / lr);
+  const size_t lds_r = sizeof(float2) * lr * n;
+  hipLaunchKernelGGL(r2c_rows_kernel, dim3(gr), dim3(SH_THREADS), lds_r, s, x, half, n, t1.log2n,
+                     R, lr, t1.tw, pred);
+  SCXR_LAUNCHED("fft r2c rows");
+  // columns of the half spectrum: n0-point transforms at stride H1
+  const AxisTables& t0 = p.ax[0];
+  Pass P{};
+  P.n = t0.n;
+  P.log2n = t0.log2n;
+  P.stride = H1;
+  P.inner = H1;
+  P.lines = p.batch * H1;
+  P.lpb = p.lpb_cols;
+  P.fused = 1;
+  P.conj_h = adj;
+  P.tw = t0.tw;
+  P.h = p.hs;
+  const unsigned gc = static_cast<unsigned>((P.lines + P.lpb - 1) / P.lpb);
+  hipLaunchKernelGGL(stockham_kernel, dim3(gc), dim3(SH_THREADS), sizeof(float2) * P.lpb * P.n, s,
+                     half, half, P, 1.0f / static_cast<float>(t0.n), pred);
+  SCXR_LAUNCHED("fft half-spectrum columns");
+  hipLaunchKernelGGL(c2r_rows_kernel, dim3(gr), dim3(SH_THREADS), lds_r, s, half, y, n, t1.log2n,
+                     R, lr, t1.tw, 1.0f / static_cast<float>(n), pred);
+  SCXR_LAUNCHED("fft c2r rows");
+  return SCXR_OK;
+}
+
 // y = real(ifftn(H' * fftn(x))) with H' = H or conj(H).
 int convolve(const Plan& p, const float* x, float* y, bool adj, const int32_t* pred,
              hipStream_t s) {
+  if (p.r2c) return convolve_r2c(p, x, y, adj, pred, s);
   if (p.nd == 1) return run_pass(p, 0, x, y, true, true, false, true, adj, pred, s);
   int rc;
   // forward along F{d-1} .. F1 (first pass reads the real input)
@@ -625,6 +781,29 @@ int scxr_cconv_plan_create(int32_t ndim, const int32_t* fshape, int64_t batch,
     rc = fail(SCXR_ERR_NOMEM, "hipMalloc of the spectrum failed");
   if (rc == SCXR_OK && hipMalloc(&p.work, sizeof(float2) * p.fsize * p.batch) != hipSuccess)
     rc = fail(SCXR_ERR_NOMEM, "hipMalloc of the FFT workspace failed");
+  if (rc == SCXR_OK && ndim == 2 && p.ax[0].log2n >= 2 && p.ax[1].log2n >= 2) {
+    p.r2c = true;
+    const long long H1 = p.F[1] / 2 + 1;
+    if (hipMalloc(&p.hs, sizeof(float2) * p.F[0] * H1) != hipSuccess)
+      rc = fail(SCXR_ERR_NOMEM, "hipMalloc of the half spectrum failed");
+    // lines per workgroup: as many as fit 8192 LDS values while the pass
+    // keeps at least 512 workgroups (SCXR_FFT_LPB="rows,cols" overrides)
+    auto pick = [](long long lines, int n) {
+      int l = std::max(1, 8192 / n);
+      while (l > 1 && (lines + l - 1) / l < 512) l /= 2;
+      return l;
+    };
+    p.lpb_rows = pick((p.batch * p.F[0] + 1) / 2, p.F[1]);
+    p.lpb_cols = pick(p.batch * H1, p.F[0]);
+    if (const char* env = std::getenv("SCXR_FFT_LPB")) {
+      int a = 0, b = 0;
+      if (std::sscanf(env, "%d,%d", &a, &b) == 2 && a >= 1 && b >= 1 &&
+          a * p.F[1] <= 8192 && b * p.F[0] <= 8192) {
+        p.lpb_rows = a;
+        p.lpb_cols = b;
+      }
+    }
+  }
   if (rc != SCXR_OK) {
     destroy(p);
     delete plan;
@@ -659,6 +838,11 @@ int scxr_cconv_set_spectrum(scxr_cconv_plan* plan, const float* h, int32_t h_is_
     hipLaunchKernelGGL(cmul_kernel, dim3(256), dim3(256), 0, s, p.fsize, p.h,
                        reinterpret_cast<const float2*>(shift), p.h);
     SCXR_LAUNCHED("cconv shift");
+  }
+  if (p.r2c) {
+    hipLaunchKernelGGL(hermitian_half_kernel, dim3(256), dim3(256), 0, s, p.h, p.hs, p.F[0],
+                       p.F[1]);
+    SCXR_LAUNCHED("cconv half spectrum");
   }
   return SCXR_OK;
 }

@@ -46,9 +46,12 @@ class SquaredL2Loss(Loss):
         """``2 scale A^T (A x - y)`` (the autodiff gradient of ``__call__``).
 
         For a :class:`CircularConvolve` the gradient is evaluated as
-        ``2 scale (G x - A^T y)`` with ``G = A^T A`` the circular
-        convolution by ``|H|^2`` and ``A^T y`` cached: one FFT convolution
-        per gradient instead of two (equal up to float32 rounding).
+        ``2 scale (G x - A^T y)`` with ``G = A^T A`` and ``A^T y`` cached: one
+        FFT convolution per gradient instead of two (equal up to float32
+        rounding).  ``A x = real(ifftn(H fftn(x)))`` is the convolution by
+        the Hermitian part ``Hs[k] = (H[k] + conj(H[-k])) / 2`` of the
+        spectrum, so ``G`` is the convolution by ``|Hs|^2`` for any ``H``
+        (``Hs = H`` for a real kernel).
         """
         from .linop import CircularConvolve
 
@@ -63,20 +66,24 @@ class SquaredL2Loss(Loss):
 
     def _conv_gram(self):
         """Cached (G, A^T y) for a circular-convolution operator; rebuilt
-        when the operator or y changes."""
+        when the operator or y changes (the cache holds the operator itself,
+        so an identity test cannot match a different, later operator)."""
         from .linop import CircularConvolve
 
-        key = (id(self.A), self.y.data_ptr(), self.y._version)
+        A = self.A
+        key = (self.y.data_ptr(), self.y._version)
         cache = getattr(self, "_gram_cache", None)
-        if cache is None or cache[0] != key:
-            A = self.A
-            hd = dev.to_numpy(A.h_dft).astype(np.complex64)
-            g = (hd.real.astype(np.float64) ** 2 + hd.imag.astype(np.float64) ** 2)
+        if cache is None or cache[0] is not A or cache[1] != key:
+            hd = dev.to_numpy(A.h_dft).astype(np.complex128)
+            axes = tuple(range(hd.ndim))
+            hneg = np.roll(np.flip(hd, axes), 1, axes)  # H[-k]
+            hs = 0.5 * (hd + np.conj(hneg))
+            g = hs.real**2 + hs.imag**2
             G = CircularConvolve(g.astype(np.complex64), A.input_shape, ndims=A.ndims,
                                  h_is_dft=True)
-            cache = (key, G, A.adj(self.y))
+            cache = (A, key, G, A.adj(self.y))
             self._gram_cache = cache
-        return cache[1], cache[2]
+        return cache[2], cache[3]
 
     @property
     def hessian(self) -> LinearOperator:

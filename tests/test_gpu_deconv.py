@@ -81,6 +81,50 @@ def test_circconv_center_and_from_operator(gpu):
         CircularConvolve(h2, (12, 8), h_is_dft=True, h_center=[1, 1])
 
 
+@pytest.mark.parametrize("x_shape,kind", [((4, 4), "real"), ((64, 256), "real"),
+                                          ((256, 64), "centre"), ((3, 16, 32), "real"),
+                                          ((8, 4096), "complex"), ((2048, 8), "complex"),
+                                          ((128, 128), "complex"), ((32, 64), "centre")])
+def test_circconv_real_2d_path(gpu, x_shape, kind):
+    """Two power-of-two axes run on half spectra (two real rows per complex
+    row transform, Hermitian part of the spectrum): real(ifft2(H fft2(x)))
+    for a real kernel, a kernel with a non-integer centre (a spectrum that
+    is not Hermitian) and an arbitrary complex spectrum (h_is_dft), forward
+    and adjoint, against the oracle (complex64 numpy FFTs) and float64."""
+    rng = np.random.default_rng(11)
+    F = x_shape[-2:]
+    x = rng.standard_normal(x_shape).astype(np.float32)
+    if kind == "complex":
+        h = (rng.standard_normal(F) + 1j * rng.standard_normal(F)).astype(np.complex64)
+        A = CircularConvolve(h, x_shape, ndims=2, h_is_dft=True)
+        hd = h
+    else:
+        h = rng.standard_normal((min(3, F[0]), min(5, F[1]))).astype(np.float32)
+        center = [-1.0, 2.5] if kind == "centre" else None
+        A = CircularConvolve(h, x_shape, ndims=2, h_center=center)
+        hd = S.circconv_hdft(h, F, center=center)
+    y = A(dev(x)).cpu().numpy()
+    yt = A.T(dev(x)).cpu().numpy()
+    xs = x.reshape(-1, *F)
+    ref = np.stack([S.circconv(v, hd) for v in xs]).reshape(x_shape)
+    reft = np.stack([S.circconv_adj(v, hd) for v in xs]).reshape(x_shape)
+    hd64 = hd.astype(np.complex128)
+    r64 = np.stack([np.fft.ifft2(hd64 * np.fft.fft2(v.astype(np.float64))).real
+                    for v in xs]).reshape(x_shape)
+    e, et, e64 = rel_l2(y, ref), rel_l2(yt, reft), rel_l2(y, r64)
+    print(f"  real 2D path {x_shape} {kind}: fwd {e:.2e} adj {et:.2e} (vs float64 {e64:.2e})",
+          flush=True)
+    assert e <= 1e-5 and et <= 1e-5 and e64 <= 1e-5
+    # dot test, normalised by the norms (with an arbitrary complex spectrum
+    # y^T A x is a sum of terms of random sign, far below ||y|| ||A x||)
+    v = rng.standard_normal(x_shape).astype(np.float32)
+    u1 = float(np.sum(v.astype(np.float64) * y))
+    u2 = float(np.sum(A.T(dev(v)).cpu().numpy().astype(np.float64) * x))
+    assert abs(u1 - u2) <= 1e-6 * np.linalg.norm(v) * np.linalg.norm(y.astype(np.float64))
+    if kind != "complex":
+        assert valid_adjoint(A, A.T, eps=1e-5)
+
+
 def test_circconv_1024_psf(gpu):
     """1024^2 image, 15x15 Gaussian PSF (sigma 2) centred at (7, 7):
     forward / adjoint against the oracle (complex64 FFTs) and float64."""
@@ -196,3 +240,27 @@ def test_circular_convolve_solver(gpu, extra_axis, center):
     got = s_dft.subproblem_solver.lhs_dft
     assert np.abs(got.imag).max() < 1e-5
     np.testing.assert_allclose(got.real, ref, rtol=0, atol=2e-6)
+
+
+@pytest.mark.parametrize("kind", ["real", "complex"])
+def test_squared_l2_grad_circconv(gpu, kind):
+    """SquaredL2Loss.grad with a CircularConvolve (evaluated as G x - A^T y
+    with G the convolution by |Hs|^2, Hs the Hermitian part of the
+    spectrum) equals A^T (A x - y), also for an arbitrary complex spectrum;
+    a new operator assigned to the loss rebuilds the cached G."""
+    rng = np.random.default_rng(5)
+    F = (64, 32)
+    if kind == "complex":
+        h = (rng.standard_normal(F) + 1j * rng.standard_normal(F)).astype(np.complex64)
+        A = CircularConvolve(h, F, h_is_dft=True)
+    else:
+        A = CircularConvolve(rng.standard_normal((3, 4)).astype(np.float32), F)
+    y = dev(rng.standard_normal(F).astype(np.float32))
+    x = dev(rng.standard_normal(F).astype(np.float32))
+    f = loss.SquaredL2Loss(y=y, A=A)
+    ref = A.adj(A(x) - y)
+    assert rel_l2(f.grad(x).cpu().numpy(), ref.cpu().numpy()) <= 1e-5
+    B = CircularConvolve(rng.standard_normal((2, 2)).astype(np.float32), F)
+    f.A = B
+    refB = B.adj(B(x) - y)
+    assert rel_l2(f.grad(x).cpu().numpy(), refB.cpu().numpy()) <= 1e-5
