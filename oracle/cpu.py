@@ -22,6 +22,13 @@ def lib():
     return _lib
 
 
+def use_library(path) -> None:
+    """Load the C restatement from ``path`` (e.g. a -march=native build)
+    for the following calls."""
+    global _lib
+    _lib = _cbuild.load(path)
+
+
 def _geom_args(g):
     x0 = np.ascontiguousarray(np.asarray(g["x0"], dtype=np.float64))
     dx = np.ascontiguousarray(np.asarray(g["dx"], dtype=np.float64))
@@ -59,6 +66,30 @@ def back_project(y, g):
     out = np.empty((M, N), np.float32)
     lib().cpu_xr2d_adjoint(M, N, A, D, x0.ctypes.data, dx.ctypes.data, float(g["y0"]),
                            ang.ctypes.data, y.ctypes.data, out.ctypes.data)
+    return out
+
+
+def project_f64acc(im, g):
+    """:func:`project` with float64 accumulation of each bin."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    M, N = im.shape
+    A, D = len(g["angles"]), g["ny"]
+    x0, dx, ang = _geom_args(g)
+    y = np.empty((A, D), np.float32)
+    lib().cpu_xr2d_forward_f64acc(M, N, A, D, x0.ctypes.data, dx.ctypes.data, float(g["y0"]),
+                                  ang.ctypes.data, im.ctypes.data, y.ctypes.data)
+    return y
+
+
+def back_project_f64acc(y, g):
+    """:func:`back_project` with float64 accumulation of each pixel."""
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    M, N = g["nx"]
+    A, D = y.shape
+    x0, dx, ang = _geom_args(g)
+    out = np.empty((M, N), np.float32)
+    lib().cpu_xr2d_adjoint_f64acc(M, N, A, D, x0.ctypes.data, dx.ctypes.data, float(g["y0"]),
+                                  ang.ctypes.data, y.ctypes.data, out.ctypes.data)
     return out
 
 

@@ -126,3 +126,58 @@ int cpu_xr2d_adjoint(int M, int N, int A, int D, const double* x0, const double*
   free(v);
   return 0;
 }
+
+/* The same projections with float64 accumulation (each sum rounded to
+ * float32 once): another valid summation order of the reference's sums,
+ * used to size the rounding-noise floor of whole ADMM iterations. */
+int cpu_xr2d_forward_f64acc(int M, int N, int A, int D, const double* x0, const double* dx,
+                            double y0, const float* angles, const float* im, float* sino) {
+  view_t* v = (view_t*)malloc(sizeof(view_t) * A);
+  if (!v) return -1;
+  views(A, angles, x0, dx, y0, v);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int a = 0; a < A; ++a) {
+    double* y = (double*)calloc((size_t)D, sizeof(double));
+    if (!y) {
+      bad = 1;
+      continue;
+    }
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < N; ++j) {
+        int ind;
+        float w;
+        weight(&v[a], (float)i, (float)j, &ind, &w);
+        const float p = im[(size_t)i * N + j];
+        if (ind >= 0 && ind < D) y[ind] += (double)(p * w);
+        if (ind + 1 >= 0 && ind + 1 < D) y[ind + 1] += (double)(p * (1.0f - w));
+      }
+    for (int b = 0; b < D; ++b) sino[(size_t)a * D + b] = (float)y[b];
+    free(y);
+  }
+  free(v);
+  return bad ? -1 : 0;
+}
+
+int cpu_xr2d_adjoint_f64acc(int M, int N, int A, int D, const double* x0, const double* dx,
+                            double y0, const float* angles, const float* sino, float* im) {
+  view_t* v = (view_t*)malloc(sizeof(view_t) * A);
+  if (!v) return -1;
+  views(A, angles, x0, dx, y0, v);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < N; ++j) {
+      double s0 = 0.0, s1 = 0.0;
+      for (int a = 0; a < A; ++a) {
+        int ind;
+        float w;
+        weight(&v[a], (float)i, (float)j, &ind, &w);
+        const float* y = sino + (size_t)a * D;
+        if (ind >= 0 && ind < D) s0 += (double)(y[ind] * w);
+        if (ind + 1 >= 0 && ind + 1 < D) s1 += (double)(y[ind + 1] * (1.0f - w));
+      }
+      im[(size_t)i * N + j] = (float)(s0 + s1);
+    }
+  free(v);
+  return 0;
+}
