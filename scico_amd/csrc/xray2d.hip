@@ -399,9 +399,11 @@ __device__ __forceinline__ void fwd_phase(float* acc, const int (&row)[CH][V],
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const float P = COLFAST ? __fadd_rn(tl, t[c][v]) : __fadd_rn(t[c][v], tl);
-        if (FR) {  // every Px >= 1: the fraction instruction (see weight_frac)
+        if (FR) {  // every pixel's Px >= 1: the fraction instruction (see weight_frac)
           w[c] = weight_frac(P, rw[v]);
-          k[c] = row[c][v] + cvt_flr_i32(fminf(P, Dmax));
+          // lanes past the line end (value 0) can sit anywhere: the bin is
+          // clamped to the padded row [-2, D] as on the floor path
+          k[c] = row[c][v] + cvt_flr_i32(__builtin_amdgcn_fmed3f(P, -2.0f, Dmax));
         } else {
           const float fl = floorf(P);
           w[c] = div_rn_sat(__fsub_rn(1.0f, __fsub_rn(P, fl)), rw[v]);
