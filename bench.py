@@ -55,6 +55,8 @@ TRAFFIC_BYTES = {
 # VALU instructions (lane-instructions) per (pixel, view) interaction,
 # rocprofv3 --pmc SQ_INSTS_VALU x 64 / interactions, same workload.
 VALU_PER_INTERACTION = {"forward": 16.23, "adjoint": 13.25}
+# the profile these counter readings come from (commands and raw figures)
+COUNTERS_FROM = "profiles/xr2d_projector_512x512_720views_det724.md"
 
 
 def log(*a):
@@ -535,6 +537,9 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6),
         "traffic": round(TRAFFIC_BYTES[dom]),
+        # traffic and instructions per interaction are counter readings of a
+        # separate rocprofv3 --pmc run (constants above), not of this run
+        "counters_from": COUNTERS_FROM,
         "algorithmic_bytes": bytes_per,
         "avg_launch_us": {k: round(v * 1e6, 2) for k, v in kt.items()},
         "back_to_back_us": {k: round(v * 1e6, 2) for k, v in kb.items()},

@@ -532,6 +532,203 @@ __global__ __launch_bounds__(SH_THREADS) void c2r_rows_kernel(const float2* __re
   }
 }
 
+// ---------------------------------------- one wave per 1024-point line --
+//
+// A 1024-point line is held by one wave, 16 values per lane, and
+// transformed with two LDS exchanges: 1024 = 16 x 64 and 64 = 16 x 4.
+// On entry lane l holds v[r] = x[l + 64 r].
+//   A: 16-point DFT over r, twiddle by W^(l k2)          (W = exp(-2 pi i / 1024))
+//   exchange: lane m = 4 k2 + a takes u[b] = Y[a + 4 b][k2]
+//   B: 16-point DFT over b, twiddle by W^(16 a kb)
+//   exchange: lane m = 4 k2 + q takes, for kb = 4 q + j, the four a's
+//   C: 4-point DFTs over a
+// On exit lane m = 4 k2 + q holds v[4 j + ka] = X[k2 + 64 q + 16 j + 256 ka].
+// Every twiddle a lane needs is known from its lane index, so the kernels
+// load them (and the spectrum values of the fused pass) before the line
+// itself, and their latency hides behind the line's.  One line per
+// 64-thread workgroup: the passes are latency bound, not throughput bound.
+constexpr int W1K = 1024;
+constexpr int WX_S = 68;                    // padded row (float2) of the exchanges
+constexpr int WX_LDS = 16 * WX_S;           // exchange buffer (float2)
+constexpr int WN_LDS = W1K + W1K / 16 + 16;  // padded natural-order buffer (float2)
+__device__ __forceinline__ int wn_pad(int k) { return k + (k >> 4); }
+
+// Output position of register i of lane m.
+__device__ __forceinline__ int w1k_out(int m, int i) {
+  return (m >> 2) + 64 * (m & 3) + 16 * (i >> 2) + 256 * (i & 3);
+}
+
+struct W1kTw {
+  float2 t1[16], t2[16];
+};
+
+__device__ __forceinline__ void w1k_twiddles(W1kTw& t, const float2* __restrict__ tw) {
+  const int l = threadIdx.x & 63, a = l & 3;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    t.t1[k] = tw[(l * k) & (W1K - 1)];
+    t.t2[k] = tw[(16 * a * k) & (W1K - 1)];
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void w1k_fft(float2 (&v)[16], float2* xb, const W1kTw& t) {
+  const int m = threadIdx.x & 63;
+  const int k2 = m >> 2, a = m & 3;
+  dft_small<16>(v, INV);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], INV ? conjf2(t.t1[k]) : t.t1[k]);
+  __syncthreads();  // xb free (previous use done)
+#pragma unroll
+  for (int k = 0; k < 16; ++k) xb[k * WX_S + m] = v[k];  // Y[l = m][k2 = k]
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < 16; ++b) v[b] = xb[k2 * WX_S + a + 4 * b];
+  dft_small<16>(v, INV);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], INV ? conjf2(t.t2[k]) : t.t2[k]);
+  __syncthreads();
+#pragma unroll
+  for (int kb = 0; kb < 16; ++kb) xb[k2 * WX_S + 4 * kb + a] = v[kb];  // Z_a[kb]
+  __syncthreads();
+  const int q = a;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float2 z[4];
+#pragma unroll
+    for (int aa = 0; aa < 4; ++aa) z[aa] = xb[k2 * WX_S + 4 * (4 * q + j) + aa];
+    dft_small<4>(z, INV);
+#pragma unroll
+    for (int ka = 0; ka < 4; ++ka) v[4 * j + ka] = z[ka];
+  }
+}
+
+// Natural distribution (lane l holds x[l + 64 r]) of values given in the
+// output order of w1k_fft, through the padded natural-order buffer.
+__device__ __forceinline__ void w1k_to_natural(float2 (&v)[16], float2* nb) {
+  const int m = threadIdx.x & 63;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; ++i) nb[wn_pad(w1k_out(m, i))] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = nb[wn_pad(m + 64 * r)];
+}
+
+// Forward rows, two real rows per wave -> half spectra (R rows of 513).
+__global__ __launch_bounds__(64) void w1k_r2c_kernel(const float* __restrict__ x,
+                                                     float2* __restrict__ half, long long R,
+                                                     const float2* __restrict__ tw,
+                                                     const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float2 xb[WX_LDS];
+  __shared__ float2 nb[WN_LDS];
+  const int l = threadIdx.x;
+  const long long ra = 2ll * blockIdx.x, rb = ra + 1;
+  W1kTw t;
+  w1k_twiddles(t, tw);
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const float a = x[ra * W1K + l + 64 * r];
+    const float b = rb < R ? x[rb * W1K + l + 64 * r] : 0.0f;
+    v[r] = make_float2(a, b);
+  }
+  w1k_fft<false>(v, xb, t);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; ++i) nb[wn_pad(w1k_out(l, i))] = v[i];
+  __syncthreads();
+  constexpr int H1 = W1K / 2 + 1;
+  for (int k = l; k < H1; k += 64) {
+    const float2 zk = nb[wn_pad(k)], zm = nb[wn_pad((W1K - k) & (W1K - 1))];
+    half[ra * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.x, zm.x)),
+                                    __fmul_rn(0.5f, __fsub_rn(zk.y, zm.y)));
+    if (rb < R)
+      half[rb * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.y, zm.y)),
+                                      __fmul_rn(0.5f, __fsub_rn(zm.x, zk.x)));
+  }
+}
+
+// Columns of the half spectrum (F0 = 1024): forward, times hs (conj for
+// the adjoint), inverse, times 1/1024; one column per wave.
+__global__ __launch_bounds__(64) void w1k_cols_kernel(float2* __restrict__ half,
+                                                      const float2* __restrict__ hs,
+                                                      long long ncols, int H1, int conj_h,
+                                                      const float2* __restrict__ tw,
+                                                      const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float2 xb[WX_LDS];
+  __shared__ float2 nb[WN_LDS];
+  const int l = threadIdx.x;
+  // consecutive workgroups go to the 8 XCDs in turn: give each XCD a run
+  // of neighbouring columns, which share cache lines
+  long long L = blockIdx.x;
+  {
+    const long long G = gridDim.x, per = (G + 7) / 8;
+    const long long cand = (L % 8) * per + L / 8;
+    if (G % 8 == 0) L = cand;
+  }
+  const long long item = L / H1;
+  const int c = static_cast<int>(L - item * H1);
+  float2* col = half + item * static_cast<long long>(W1K) * H1 + c;
+  W1kTw t;
+  w1k_twiddles(t, tw);
+  float2 h[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const float2 hv = hs[static_cast<long long>(w1k_out(l, i)) * H1 + c];
+    h[i] = conj_h ? conjf2(hv) : hv;
+  }
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = col[static_cast<long long>(l + 64 * r) * H1];
+  w1k_fft<false>(v, xb, t);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = cmul(h[i], v[i]);
+  w1k_to_natural(v, nb);
+  w1k_fft<true>(v, xb, t);
+  constexpr float sc = 1.0f / W1K;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    col[static_cast<long long>(w1k_out(l, i)) * H1] =
+        make_float2(__fmul_rn(v[i].x, sc), __fmul_rn(v[i].y, sc));
+}
+
+// Inverse rows: half spectra -> two real rows per wave, times 1/1024.
+__global__ __launch_bounds__(64) void w1k_c2r_kernel(const float2* __restrict__ half,
+                                                     float* __restrict__ y, long long R,
+                                                     const float2* __restrict__ tw,
+                                                     const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float2 xb[WX_LDS];
+  __shared__ float2 nb[WN_LDS];
+  const int l = threadIdx.x;
+  const long long ra = 2ll * blockIdx.x, rb = ra + 1;
+  W1kTw t;
+  w1k_twiddles(t, tw);
+  constexpr int H1 = W1K / 2 + 1, nh = W1K / 2;
+  for (int k = l; k < H1; k += 64) {
+    float2 a = half[ra * H1 + k];
+    float2 b = rb < R ? half[rb * H1 + k] : make_float2(0.0f, 0.0f);
+    if (k == 0 || k == nh) a.y = b.y = 0.0f;
+    nb[wn_pad(k)] = make_float2(__fsub_rn(a.x, b.y), __fadd_rn(a.y, b.x));
+    if (k > 0 && k < nh) nb[wn_pad(W1K - k)] = make_float2(__fadd_rn(a.x, b.y), __fsub_rn(b.x, a.y));
+  }
+  __syncthreads();
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = nb[wn_pad(l + 64 * r)];
+  w1k_fft<true>(v, xb, t);
+  constexpr float sc = 1.0f / W1K;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int k = w1k_out(l, i);
+    y[ra * W1K + k] = __fmul_rn(v[i].x, sc);
+    if (rb < R) y[rb * W1K + k] = __fmul_rn(v[i].y, sc);
+  }
+}
+
 // Hermitian part of the (F0, F1) spectrum on the half grid (F0, F1/2 + 1).
 __global__ __launch_bounds__(256) void hermitian_half_kernel(const float2* __restrict__ h,
                                                              float2* __restrict__ hs, int F0,
@@ -685,6 +882,18 @@ int convolve_r2c(const Plan& p, const float* x, float* y, bool adj, const int32_
   const long long R = p.batch * p.F[0];
   float2* half = p.work;  // R rows of H1 (fits in the batch * fsize workspace)
   const long long cl = (R + 1) / 2;  // complex row transforms
+  if (n == W1K && p.F[0] == W1K) {
+    hipLaunchKernelGGL(w1k_r2c_kernel, dim3(static_cast<unsigned>(cl)), dim3(64), 0, s, x, half,
+                       R, t1.tw, pred);
+    SCXR_LAUNCHED("fft r2c rows (wave)");
+    hipLaunchKernelGGL(w1k_cols_kernel, dim3(static_cast<unsigned>(p.batch * H1)), dim3(64), 0, s,
+                       half, p.hs, p.batch * H1, H1, static_cast<int>(adj), p.ax[0].tw, pred);
+    SCXR_LAUNCHED("fft half-spectrum columns (wave)");
+    hipLaunchKernelGGL(w1k_c2r_kernel, dim3(static_cast<unsigned>(cl)), dim3(64), 0, s, half, y,
+                       R, t1.tw, pred);
+    SCXR_LAUNCHED("fft c2r rows (wave)");
+    return SCXR_OK;
+  }
   int lr = p.lpb_rows;
   const unsigned gr = static_cast<unsigned>((cl + lr - 1) / lr);
   const size_t lds_r = sizeof(float2) * lr * n;
