@@ -64,6 +64,11 @@ int scxr_device_info(int32_t* cu_count, int32_t* warp_size, int64_t* lds_bytes,
  */
 typedef struct scxr_xr2d_plan scxr_xr2d_plan;
 
+/* The tiled kernels hold a view group's detector rows in LDS and give each
+ * lane a disjoint detector footprint; a geometry beyond that (a detector
+ * too long for LDS, projected pixels much narrower or much wider than a
+ * detector bin) gets general-geometry kernels with the same bins and
+ * weights (see scxr_xr2d_info). */
 int scxr_xr2d_plan_create(int32_t M, int32_t N, int32_t A, int32_t D,
                           const double* x0 /* host, 2 */,
                           const double* dx /* host, 2 */, double y0,
@@ -98,7 +103,12 @@ int scxr_ramp_filter(int32_t A, int32_t D, int32_t Dout, int32_t shift, const fl
 /* Launch facts: out[0] forward workgroups (= partial count of
  * scxr_xr2d_forward_res), out[1] adjoint workgroups (= partial count of the
  * adjoint dot epilogue), out[2] views per forward workgroup, out[3] adjoint
- * detector window. */
+ * detector window.  A geometry outside the tiled kernels' limits (see
+ * scxr_xr2d_plan_create) runs general-geometry kernels instead; then out[0]
+ * (forward) or out[1] (adjoint) is 0, and scxr_xr2d_forward_res and
+ * scxr_xr2d_adjoint_epi with an epilogue or dot partials return
+ * SCXR_ERR_UNSUPPORTED for that plan; scxr_xr2d_forward and
+ * scxr_xr2d_adjoint work as for any other. */
 int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out /* host, 4 */);
 
 /* Forward projection that also writes, per forward workgroup, the partial
@@ -282,7 +292,9 @@ int scxr_tv_norm(int32_t ndim, const int32_t* shape, const float* x, int32_t iso
  * for real float32 input of shape (batch..., F0[, F1[, F2]]).  FFT lengths:
  * powers of two up to 4096 (radix-2 in LDS) or any length up to 512
  * (direct DFT).  The plan owns the spectrum (complex64, F-shaped) and a
- * complex workspace of batch * prod(F).
+ * complex workspace of batch * prod(F).  Two power-of-two axes convolve on
+ * half spectra (F0 x (F1/2 + 1)) with the Hermitian part of the spectrum,
+ * which equals real(ifftn(H * fftn(x))) for any H.
  */
 typedef struct scxr_cconv_plan scxr_cconv_plan;
 int scxr_cconv_plan_create(int32_t ndim, const int32_t* fshape /* host */, int64_t batch,

@@ -328,24 +328,31 @@ def admm_tv(y, A, AT, lam, rho, x0, maxiter, cg_tol=1e-4, cg_maxiter=100, circul
 # ------------------------------------------------------------------ APGM --
 
 
-def apgm(grad_f, prox_g, L0, x0, maxiter):
+def apgm(grad_f, prox_g, L0, x0, maxiter, objective=None):
     """``AcceleratedPGM`` with constant step ``L0`` (``_pgm.py:221-240``).
 
-    Returns ``(x, residuals)``.
+    Returns ``(x, residuals)``, or ``(x, residuals, objectives)`` with
+    ``objective(x)`` evaluated after every iteration, as the reference's
+    iteration statistics do (``_common.py:306-312``, ``_pgm.py:151-156``).
     """
     x = np.asarray(x0, dtype=f32)
     v = x
-    t = 1.0
+    t = f32(1.0)
     L = f32(L0)
-    res = []
+    res, obj = [], []
     for _ in range(maxiter):
         x_old = x
         x = prox_g((v - f32(1.0 / L) * grad_f(v)).astype(f32), f32(1.0 / L))
         res.append(float(np.linalg.norm((x - v).astype(np.float64))))
+        if objective is not None:
+            obj.append(float(objective(x)))
+        # the momentum sequence is float32 (jax_enable_x64 off): t_old ** 2,
+        # 4 *, 1 +, sqrt, 1 +, 0.5 * and (t_old - 1) / t each round
         t_old = t
-        t = 0.5 * (1 + np.sqrt(1 + 4 * t_old**2))
-        v = (x + f32((t_old - 1) / t) * (x - x_old)).astype(f32)
-    return x, res
+        t = f32(0.5) * (f32(1) + np.sqrt(f32(1) + f32(4) * (t_old * t_old)))
+        c = (t_old - f32(1)) / t
+        v = (x + c * (x - x_old)).astype(f32)
+    return (x, res) if objective is None else (x, res, obj)
 
 
 # -------------------------------------------------------- proximal ADMM --

@@ -27,6 +27,12 @@ from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 f32 = np.float32
+# Bars of the float32 GPU gradient against float64 central differences
+# (64 directions, eps 1e-5; errors relative to |FD| floored at a tenth of
+# its median): about 3x the errors measured on MI355X (median 1.8e-4,
+# largest 9.9e-3).
+GRAD_BAR_MEDIAN = 5e-4
+GRAD_BAR_MAX = 3e-2
 
 
 def dev(a):
@@ -151,36 +157,50 @@ def test_checkpoint_spacing_invariant(gpu):
 
 
 def test_unrolled_gradient_vs_float64_oracle(gpu):
-    """Directional derivatives of l = ||x_K - x_true||^2 against central
-    differences of the oracle's float64 unrolled reconstruction."""
-    N, V, K, lam = 24, 30, 20, 0.05
-    H, xgt, y = _problem(N, V)
-    L = _lipschitz(H)
-    g = _geom(H)
-    A64 = lambda v: X.project_f64(v, g["x0"], g["dx"], g["y0"], g["ny"], g["angles"])
-    AT64 = lambda v: X.back_project_f64(v, g["x0"], g["dx"], g["nx"], g["y0"], g["angles"])
-    rng = np.random.default_rng(5)
-    w0 = rng.uniform(0.5, 1.5, (N, N))
+    """Directional derivatives of l = ||x_K - x_true||^2 with respect to the
+    sinogram and the TV weight map, at 64^2 / 60 views / K = 20 with
+    checkpoints every 5 steps, against central differences (eps 1e-3, 64
+    random directions) of the oracle's float64 unrolled reconstruction,
+    precomputed by ``tests/golden/make_grad_fixture.py``."""
+    import os
+    import sys
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    if gold not in sys.path:
+        sys.path.insert(0, gold)
+    import make_grad_fixture as G
+
+    fx = np.load(os.path.join(gold, "unrolled_grad_fd.npz"), allow_pickle=False)
+    g, xgt, y, w0, _, _ = G.problem()
+    L = float(fx["L"])
+    H = XRayTransform2D((G.N, G.N), g["angles"])
     yy = dev(y).requires_grad_(True)
-    ww = dev(w0.astype(f32)).requires_grad_(True)
-    x = AG.unrolled_apgm_tv(H, yy, ww, L, lam, K=K)
-    ((x - dev(xgt)) ** 2).sum().backward()
+    ww = dev(w0).requires_grad_(True)
+    x = AG.unrolled_apgm_tv(H, yy, ww, L, G.LAM, K=G.K, checkpoint_every=5)
+    lval = ((x.double() - dev(xgt).double()) ** 2).sum()
+    lval.backward()
+    lval = float(lval.detach())
     gy, gw = yy.grad.cpu().numpy().astype(np.float64), ww.grad.cpu().numpy().astype(np.float64)
-
-    def loss64(yv, wv):
-        xk = S.unrolled_apgm_tv(yv, A64, AT64, L, lam, wv, K, np.zeros((N, N)), dtype=np.float64)
-        return float(np.sum((xk - xgt) ** 2))
-
-    errs = []
-    eps = 1e-3
-    for _ in range(8):
-        dy = rng.standard_normal(y.shape)
-        dw = rng.standard_normal(w0.shape)
-        fd = (loss64(y + eps * dy, w0 + eps * dw) - loss64(y - eps * dy, w0 - eps * dw)) / (2 * eps)
-        an = np.vdot(gy, dy) + np.vdot(gw, dw)
-        errs.append(abs(an - fd) / max(abs(fd), 1e-12))
-    print(f"  unrolled APGM-TV grad vs float64 FD: max rel err {max(errs):.2e}", flush=True)
-    assert max(errs) <= 1e-2
+    dirs = G.directions(y.shape, float(np.abs(y).max()))
+    an = np.array([np.vdot(gy, dy) + np.vdot(gw, dw) for dy, dw in dirs])
+    fd3, fd3b, fd5 = fx["fd"]
+    # errors relative to |FD|, floored at a tenth of the median |FD| over the
+    # directions (a direction almost orthogonal to the gradient has FD ~ 0)
+    scale = np.maximum(np.abs(fd5), 0.1 * np.median(np.abs(fd5)))
+    err5 = np.abs(an - fd5) / scale
+    err3 = np.abs(an - fd3) / scale
+    # error of each eps = 1e-3 central difference (kinks of the TV prox)
+    fd_err3 = np.abs(fd3 - fd3b) / scale
+    print(f"  unrolled APGM-TV gradient (64^2/60, K 20, ckpt 5) vs float64 central differences: "
+          f"loss {lval:.6e} vs {float(fx['loss0']):.6e}; eps 1e-5: rel err median "
+          f"{np.median(err5):.2e}, 90th pct {np.percentile(err5, 90):.2e}, max {err5.max():.2e}; "
+          f"eps 1e-3: median {np.median(err3):.2e}, max {err3.max():.2e} (its own central-"
+          f"difference error up to {fd_err3.max():.2e})", flush=True)
+    assert abs(lval - float(fx["loss0"])) <= 1e-4 * float(fx["loss0"])
+    assert np.median(err5) <= GRAD_BAR_MEDIAN and err5.max() <= GRAD_BAR_MAX
+    # at the protocol's step, each direction within its central difference's
+    # own error (3x the eps 1e-3 / 5e-4 disagreement) plus the fine-step bar
+    assert np.all(err3 <= 3 * fd_err3 + GRAD_BAR_MAX)
 
 
 def test_unrolled_gradient_central_fd_512(gpu):
