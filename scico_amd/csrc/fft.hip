@@ -16,6 +16,13 @@
 // inverse passes on F1 .. F{d-1}; the first pass reads the real input and
 // the last one writes the real part.  With d == 1 the single fused pass
 // does both.
+//
+// Real convolution over two power-of-two axes (the deconvolution workload)
+// runs on half spectra instead: rows transformed two real rows per complex
+// FFT into (F1/2 + 1)-wide half spectra, one fused pass on those columns
+// with the Hermitian part of H, rows inverted back to real.  At 1024 x 1024
+// each line is one 256-thread workgroup's registers (four values per
+// thread, radix-4 Stockham stages exchanged through LDS).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -357,13 +364,11 @@ __global__ __launch_bounds__(SH_THREADS) void stockham_kernel(const void* __rest
   // the 8 XCDs in turn) take consecutive line groups, so the cache lines
   // that neighbouring columns share stay in that XCD's L2.
   long long blk = blockIdx.x;
+  // (a bijection of the first 8 * floor(G / 8) workgroups; the rest keep
+  // their own line group)
   if (!contiguous) {
-    const long long G = gridDim.x, per = (G + 7) / 8;
-    const long long x = blk % 8, q = blk / 8;
-    const long long cand = x * per + q;
-    // the last XCD's share may be short: keep the identity map when the
-    // remapped index would fall outside the grid
-    blk = (G % 8 == 0) ? cand : blk;
+    const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
+    if (blk < G8) blk = (blk % 8) * (G8 / 8) + blk / 8;
   }
   const long long line0 = blk * lpb;
   const int nl = static_cast<int>(min(static_cast<long long>(lpb), P.lines - line0));
@@ -532,116 +537,86 @@ __global__ __launch_bounds__(SH_THREADS) void c2r_rows_kernel(const float2* __re
   }
 }
 
-// ---------------------------------------- one wave per 1024-point line --
+// ------------------------------- 1024-point lines on 256 threads --------
 //
-// A 1024-point line is held by one wave, 16 values per lane, and
-// transformed with two LDS exchanges: 1024 = 16 x 64 and 64 = 16 x 4.
-// On entry lane l holds v[r] = x[l + 64 r].
-//   A: 16-point DFT over r, twiddle by W^(l k2)          (W = exp(-2 pi i / 1024))
-//   exchange: lane m = 4 k2 + a takes u[b] = Y[a + 4 b][k2]
-//   B: 16-point DFT over b, twiddle by W^(16 a kb)
-//   exchange: lane m = 4 k2 + q takes, for kb = 4 q + j, the four a's
-//   C: 4-point DFTs over a
-// On exit lane m = 4 k2 + q holds v[4 j + ka] = X[k2 + 64 q + 16 j + 256 ka].
-// Every twiddle a lane needs is known from its lane index, so the kernels
-// load them (and the spectrum values of the fused pass) before the line
-// itself, and their latency hides behind the line's.  One line per
-// 64-thread workgroup: the passes are latency bound, not throughput bound.
-constexpr int W1K = 1024;
-constexpr int WX_S = 68;                    // padded row (float2) of the exchanges
-constexpr int WX_LDS = 16 * WX_S;           // exchange buffer (float2)
-constexpr int WN_LDS = W1K + W1K / 16 + 16;  // padded natural-order buffer (float2)
-__device__ __forceinline__ int wn_pad(int k) { return k + (k >> 4); }
+// Radix-4 Stockham transform of one 1024-point line by a 256-thread
+// workgroup, four values per thread: thread j holds x[j + 256 r], r < 4,
+// on entry and X[j + 256 r] on exit (natural order, no reordering pass).
+// Stage s (Ns = 4^s, s < 5): twiddle a[r] by W_{4 Ns}^{r k}, k = j mod Ns
+// (W = exp(-2 pi i / ...)), 4-point DFT over r, write a[r] to
+// (j / Ns) 4 Ns + k + r Ns; the next stage reads positions j + 256 r.
+// Stage 0 takes the registers as loaded and stage 4's outputs stay in
+// registers, so a transform makes four LDS exchanges.  Four waves per line
+// (2052 waves for the 513 columns of a 1024 x 1024 half spectrum) hide the
+// latency that one wave per line leaves exposed.
+constexpr int Q1K = 1024, QT = 256;
+constexpr int Q_LDS = Q1K + Q1K / 16;  // padded line (float2)
+__device__ __forceinline__ int q_pad(int i) { return i + (i >> 4); }
 
-// Output position of register i of lane m.
-__device__ __forceinline__ int w1k_out(int m, int i) {
-  return (m >> 2) + 64 * (m & 3) + 16 * (i >> 2) + 256 * (i & 3);
-}
-
-struct W1kTw {
-  float2 t1[16], t2[16];
+struct QTw {
+  float2 w[4][3];  // stages 1..4, r = 1..3
 };
 
-__device__ __forceinline__ void w1k_twiddles(W1kTw& t, const float2* __restrict__ tw) {
-  const int l = threadIdx.x & 63, a = l & 3;
+__device__ __forceinline__ void q_twiddles(QTw& t, const float2* __restrict__ tw) {
+  const int j = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    t.t1[k] = tw[(l * k) & (W1K - 1)];
-    t.t2[k] = tw[(16 * a * k) & (W1K - 1)];
+  for (int s = 1; s < 5; ++s) {
+    const int Ns = 1 << (2 * s);
+    const int k = j & (Ns - 1);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) t.w[s - 1][r - 1] = tw[(r * k * (QT / Ns)) & (Q1K - 1)];
   }
 }
 
 template <bool INV>
-__device__ __forceinline__ void w1k_fft(float2 (&v)[16], float2* xb, const W1kTw& t) {
-  const int m = threadIdx.x & 63;
-  const int k2 = m >> 2, a = m & 3;
-  dft_small<16>(v, INV);
+__device__ __forceinline__ void q_fft(float2 (&a)[4], float2* lds, const QTw& t) {
+  const int j = threadIdx.x;
 #pragma unroll
-  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], INV ? conjf2(t.t1[k]) : t.t1[k]);
-  __syncthreads();  // xb free (previous use done)
+  for (int s = 0; s < 5; ++s) {
+    const int Ns = 1 << (2 * s);
+    if (s > 0) {
+      __syncthreads();  // the previous stage's writes are visible
 #pragma unroll
-  for (int k = 0; k < 16; ++k) xb[k * WX_S + m] = v[k];  // Y[l = m][k2 = k]
-  __syncthreads();
+      for (int r = 0; r < 4; ++r) a[r] = lds[q_pad(j + QT * r)];
 #pragma unroll
-  for (int b = 0; b < 16; ++b) v[b] = xb[k2 * WX_S + a + 4 * b];
-  dft_small<16>(v, INV);
+      for (int r = 1; r < 4; ++r)
+        a[r] = cmul(a[r], INV ? conjf2(t.w[s - 1][r - 1]) : t.w[s - 1][r - 1]);
+    }
+    dft_small<4>(a, INV);
+    if (s < 4) {
+      const int base = (j >> (2 * s)) * 4 * Ns + (j & (Ns - 1));
+      __syncthreads();  // every read of the buffer is done before it is overwritten
 #pragma unroll
-  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], INV ? conjf2(t.t2[k]) : t.t2[k]);
-  __syncthreads();
-#pragma unroll
-  for (int kb = 0; kb < 16; ++kb) xb[k2 * WX_S + 4 * kb + a] = v[kb];  // Z_a[kb]
-  __syncthreads();
-  const int q = a;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    float2 z[4];
-#pragma unroll
-    for (int aa = 0; aa < 4; ++aa) z[aa] = xb[k2 * WX_S + 4 * (4 * q + j) + aa];
-    dft_small<4>(z, INV);
-#pragma unroll
-    for (int ka = 0; ka < 4; ++ka) v[4 * j + ka] = z[ka];
+      for (int r = 0; r < 4; ++r) lds[q_pad(base + r * Ns)] = a[r];
+    }
   }
 }
 
-// Natural distribution (lane l holds x[l + 64 r]) of values given in the
-// output order of w1k_fft, through the padded natural-order buffer.
-__device__ __forceinline__ void w1k_to_natural(float2 (&v)[16], float2* nb) {
-  const int m = threadIdx.x & 63;
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 16; ++i) nb[wn_pad(w1k_out(m, i))] = v[i];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = nb[wn_pad(m + 64 * r)];
-}
-
-// Forward rows, two real rows per wave -> half spectra (R rows of 513).
-__global__ __launch_bounds__(64) void w1k_r2c_kernel(const float* __restrict__ x,
+// Forward rows: two real rows per complex transform -> half spectra (R rows
+// of 513).  z = x_a + i x_b; X_a[k] = (Z[k] + conj Z[-k]) / 2,
+// X_b[k] = (Z[k] - conj Z[-k]) / 2i.
+__global__ __launch_bounds__(QT) void q1k_r2c_kernel(const float* __restrict__ x,
                                                      float2* __restrict__ half, long long R,
                                                      const float2* __restrict__ tw,
                                                      const int32_t* pred) {
   if (pred_off(pred)) return;
-  __shared__ float2 xb[WX_LDS];
-  __shared__ float2 nb[WN_LDS];
-  const int l = threadIdx.x;
+  __shared__ float2 lds[Q_LDS];
+  const int j = threadIdx.x;
   const long long ra = 2ll * blockIdx.x, rb = ra + 1;
-  W1kTw t;
-  w1k_twiddles(t, tw);
-  float2 v[16];
+  QTw t;
+  q_twiddles(t, tw);
+  float2 a[4];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const float a = x[ra * W1K + l + 64 * r];
-    const float b = rb < R ? x[rb * W1K + l + 64 * r] : 0.0f;
-    v[r] = make_float2(a, b);
-  }
-  w1k_fft<false>(v, xb, t);
-  __syncthreads();
+  for (int r = 0; r < 4; ++r)
+    a[r] = make_float2(x[ra * Q1K + j + QT * r], rb < R ? x[rb * Q1K + j + QT * r] : 0.0f);
+  q_fft<false>(a, lds, t);
+  __syncthreads();  // stage 4's reads are done
 #pragma unroll
-  for (int i = 0; i < 16; ++i) nb[wn_pad(w1k_out(l, i))] = v[i];
+  for (int r = 0; r < 4; ++r) lds[q_pad(j + QT * r)] = a[r];
   __syncthreads();
-  constexpr int H1 = W1K / 2 + 1;
-  for (int k = l; k < H1; k += 64) {
-    const float2 zk = nb[wn_pad(k)], zm = nb[wn_pad((W1K - k) & (W1K - 1))];
+  constexpr int H1 = Q1K / 2 + 1;
+  for (int k = j; k < H1; k += QT) {
+    const float2 zk = lds[q_pad(k)], zm = lds[q_pad((Q1K - k) & (Q1K - 1))];
     half[ra * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.x, zm.x)),
                                     __fmul_rn(0.5f, __fsub_rn(zk.y, zm.y)));
     if (rb < R)
@@ -651,81 +626,187 @@ __global__ __launch_bounds__(64) void w1k_r2c_kernel(const float* __restrict__ x
 }
 
 // Columns of the half spectrum (F0 = 1024): forward, times hs (conj for
-// the adjoint), inverse, times 1/1024; one column per wave.
-__global__ __launch_bounds__(64) void w1k_cols_kernel(float2* __restrict__ half,
-                                                      const float2* __restrict__ hs,
-                                                      long long ncols, int H1, int conj_h,
-                                                      const float2* __restrict__ tw,
+// the adjoint), inverse, times 1/1024; one column per workgroup.
+__global__ __launch_bounds__(QT) void q1k_cols_kernel(float2* __restrict__ half,
+                                                      const float2* __restrict__ hs, int H1,
+                                                      int conj_h, const float2* __restrict__ tw,
                                                       const int32_t* pred) {
   if (pred_off(pred)) return;
-  __shared__ float2 xb[WX_LDS];
-  __shared__ float2 nb[WN_LDS];
-  const int l = threadIdx.x;
+  __shared__ float2 lds[Q_LDS];
+  const int j = threadIdx.x;
   // consecutive workgroups go to the 8 XCDs in turn: give each XCD a run
-  // of neighbouring columns, which share cache lines
+  // of neighbouring columns, which share cache lines (a bijection of the
+  // first 8 * floor(G / 8) workgroups; the rest keep their own column)
   long long L = blockIdx.x;
   {
-    const long long G = gridDim.x, per = (G + 7) / 8;
-    const long long cand = (L % 8) * per + L / 8;
-    if (G % 8 == 0) L = cand;
+    const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
+    if (L < G8) L = (L % 8) * (G8 / 8) + L / 8;
   }
   const long long item = L / H1;
   const int c = static_cast<int>(L - item * H1);
-  float2* col = half + item * static_cast<long long>(W1K) * H1 + c;
-  W1kTw t;
-  w1k_twiddles(t, tw);
-  float2 h[16];
+  float2* col = half + item * static_cast<long long>(Q1K) * H1 + c;
+  QTw t;
+  q_twiddles(t, tw);
+  float2 h[4], a[4];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const float2 hv = hs[static_cast<long long>(w1k_out(l, i)) * H1 + c];
-    h[i] = conj_h ? conjf2(hv) : hv;
+  for (int r = 0; r < 4; ++r) {
+    const float2 hv = hs[static_cast<long long>(j + QT * r) * H1 + c];
+    h[r] = conj_h ? conjf2(hv) : hv;
   }
-  float2 v[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = col[static_cast<long long>(l + 64 * r) * H1];
-  w1k_fft<false>(v, xb, t);
+  for (int r = 0; r < 4; ++r) a[r] = col[static_cast<long long>(j + QT * r) * H1];
+  q_fft<false>(a, lds, t);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = cmul(h[i], v[i]);
-  w1k_to_natural(v, nb);
-  w1k_fft<true>(v, xb, t);
-  constexpr float sc = 1.0f / W1K;
+  for (int r = 0; r < 4; ++r) a[r] = cmul(h[r], a[r]);
+  q_fft<true>(a, lds, t);
+  constexpr float sc = 1.0f / Q1K;
 #pragma unroll
-  for (int i = 0; i < 16; ++i)
-    col[static_cast<long long>(w1k_out(l, i)) * H1] =
-        make_float2(__fmul_rn(v[i].x, sc), __fmul_rn(v[i].y, sc));
+  for (int r = 0; r < 4; ++r)
+    col[static_cast<long long>(j + QT * r) * H1] =
+        make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
 }
 
-// Inverse rows: half spectra -> two real rows per wave, times 1/1024.
-__global__ __launch_bounds__(64) void w1k_c2r_kernel(const float2* __restrict__ half,
+// Inverse rows: half spectra -> two real rows per transform, times 1/1024.
+__global__ __launch_bounds__(QT) void q1k_c2r_kernel(const float2* __restrict__ half,
                                                      float* __restrict__ y, long long R,
                                                      const float2* __restrict__ tw,
                                                      const int32_t* pred) {
   if (pred_off(pred)) return;
-  __shared__ float2 xb[WX_LDS];
-  __shared__ float2 nb[WN_LDS];
-  const int l = threadIdx.x;
+  __shared__ float2 lds[Q_LDS];
+  const int j = threadIdx.x;
   const long long ra = 2ll * blockIdx.x, rb = ra + 1;
-  W1kTw t;
-  w1k_twiddles(t, tw);
-  constexpr int H1 = W1K / 2 + 1, nh = W1K / 2;
-  for (int k = l; k < H1; k += 64) {
+  QTw t;
+  q_twiddles(t, tw);
+  constexpr int H1 = Q1K / 2 + 1, nh = Q1K / 2;
+  for (int k = j; k < H1; k += QT) {
     float2 a = half[ra * H1 + k];
     float2 b = rb < R ? half[rb * H1 + k] : make_float2(0.0f, 0.0f);
     if (k == 0 || k == nh) a.y = b.y = 0.0f;
-    nb[wn_pad(k)] = make_float2(__fsub_rn(a.x, b.y), __fadd_rn(a.y, b.x));
-    if (k > 0 && k < nh) nb[wn_pad(W1K - k)] = make_float2(__fadd_rn(a.x, b.y), __fsub_rn(b.x, a.y));
+    lds[q_pad(k)] = make_float2(__fsub_rn(a.x, b.y), __fadd_rn(a.y, b.x));
+    if (k > 0 && k < nh)
+      lds[q_pad(Q1K - k)] = make_float2(__fadd_rn(a.x, b.y), __fsub_rn(b.x, a.y));
   }
   __syncthreads();
-  float2 v[16];
+  float2 a[4];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = nb[wn_pad(l + 64 * r)];
-  w1k_fft<true>(v, xb, t);
-  constexpr float sc = 1.0f / W1K;
+  for (int r = 0; r < 4; ++r) a[r] = lds[q_pad(j + QT * r)];
+  q_fft<true>(a, lds, t);
+  constexpr float sc = 1.0f / Q1K;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int k = w1k_out(l, i);
-    y[ra * W1K + k] = __fmul_rn(v[i].x, sc);
-    if (rb < R) y[rb * W1K + k] = __fmul_rn(v[i].y, sc);
+  for (int r = 0; r < 4; ++r) {
+    y[ra * Q1K + j + QT * r] = __fmul_rn(a[r].x, sc);
+    if (rb < R) y[rb * Q1K + j + QT * r] = __fmul_rn(a[r].y, sc);
+  }
+}
+
+// ---------------------------------- fused AcceleratedPGM gradient step --
+//
+// For f(x) = 0.5 ||A x - y||^2 with A the 1024 x 1024 real circular
+// convolution, one convolution per iteration serves both the gradient and
+// the objective:
+//   * G x (G = A^T A: the convolution by |Hs|^2, real on the half grid),
+//     and G v = (1 + c) G x - c G x_prev for the extrapolated point
+//     v = x + c (x - x_prev) (linearity), so u = v - (1/L) (G v - A^T y);
+//   * ||A x - y||^2 = sum_k w_k |Hs[k] X[k] - Y[k]|^2 / (F0 F1) over the
+//     half spectrum (Parseval; w_k = 1 on the first and the Nyquist column,
+//     2 elsewhere), X and Y the spectra of x and y.
+
+// Columns: forward, objective partial of the column (if obj), times gs,
+// inverse, times 1/1024.
+__global__ __launch_bounds__(QT) void q1k_cols_gram_kernel(float2* __restrict__ half,
+                                                           const float2* __restrict__ hs,
+                                                           const float* __restrict__ gs,
+                                                           const float2* __restrict__ Y,
+                                                           double* __restrict__ obj, int H1,
+                                                           const float2* __restrict__ tw,
+                                                           const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float2 lds[Q_LDS];
+  __shared__ double sh[4];
+  const int j = threadIdx.x;
+  long long L = blockIdx.x;
+  {
+    const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
+    if (L < G8) L = (L % 8) * (G8 / 8) + L / 8;
+  }
+  const int c = static_cast<int>(L);
+  float2* col = half + c;
+  QTw t;
+  q_twiddles(t, tw);
+  float g[4];
+  float2 h[4], yk[4], a[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const long long e = static_cast<long long>(j + QT * r) * H1 + c;
+    g[r] = gs[e];
+    if (obj) {
+      h[r] = hs[e];
+      yk[r] = Y[e];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) a[r] = col[static_cast<long long>(j + QT * r) * H1];
+  q_fft<false>(a, lds, t);
+  if (obj) {
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float2 p = cmul(h[r], a[r]);
+      const double dx = static_cast<double>(p.x) - static_cast<double>(yk[r].x);
+      const double dy = static_cast<double>(p.y) - static_cast<double>(yk[r].y);
+      acc += dx * dx + dy * dy;
+    }
+    acc = block_sum_f64(acc, sh);
+    if (j == 0) obj[L] = (c == 0 || c == H1 - 1) ? acc : 2.0 * acc;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) a[r] = make_float2(__fmul_rn(g[r], a[r].x), __fmul_rn(g[r], a[r].y));
+  q_fft<true>(a, lds, t);
+  constexpr float sc = 1.0f / Q1K;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    col[static_cast<long long>(j + QT * r) * H1] =
+        make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
+}
+
+// Inverse rows into G x, then the gradient step:
+//   gv = (1 + c) gx_new - c gx_prev,  u = v + b (gv - aty)   (b = -1/L)
+// with gx <- gx_new.
+__global__ __launch_bounds__(QT) void q1k_c2r_step_kernel(
+    const float2* __restrict__ half, float* __restrict__ gx, const float* __restrict__ v,
+    const float* __restrict__ aty, float c1, float c, float b, float* __restrict__ u,
+    const float2* __restrict__ tw, const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float2 lds[Q_LDS];
+  const int j = threadIdx.x;
+  const long long ra = 2ll * blockIdx.x, rb = ra + 1;
+  QTw t;
+  q_twiddles(t, tw);
+  constexpr int H1 = Q1K / 2 + 1, nh = Q1K / 2;
+  for (int k = j; k < H1; k += QT) {
+    float2 a = half[ra * H1 + k];
+    float2 bb = half[rb * H1 + k];
+    if (k == 0 || k == nh) a.y = bb.y = 0.0f;
+    lds[q_pad(k)] = make_float2(__fsub_rn(a.x, bb.y), __fadd_rn(a.y, bb.x));
+    if (k > 0 && k < nh)
+      lds[q_pad(Q1K - k)] = make_float2(__fadd_rn(a.x, bb.y), __fsub_rn(bb.x, a.y));
+  }
+  __syncthreads();
+  float2 a[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) a[r] = lds[q_pad(j + QT * r)];
+  q_fft<true>(a, lds, t);
+  constexpr float sc = 1.0f / Q1K;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const long long e = (h ? rb : ra) * Q1K + j + QT * r;
+      const float gn = __fmul_rn(h ? a[r].y : a[r].x, sc);
+      const float gv = __fsub_rn(__fmul_rn(c1, gn), __fmul_rn(c, gx[e]));
+      u[e] = __fadd_rn(v[e], __fmul_rn(b, __fsub_rn(gv, aty[e])));
+      gx[e] = gn;
+    }
   }
 }
 
@@ -882,16 +963,16 @@ int convolve_r2c(const Plan& p, const float* x, float* y, bool adj, const int32_
   const long long R = p.batch * p.F[0];
   float2* half = p.work;  // R rows of H1 (fits in the batch * fsize workspace)
   const long long cl = (R + 1) / 2;  // complex row transforms
-  if (n == W1K && p.F[0] == W1K) {
-    hipLaunchKernelGGL(w1k_r2c_kernel, dim3(static_cast<unsigned>(cl)), dim3(64), 0, s, x, half,
+  if (n == Q1K && p.F[0] == Q1K) {
+    hipLaunchKernelGGL(q1k_r2c_kernel, dim3(static_cast<unsigned>(cl)), dim3(QT), 0, s, x, half,
                        R, t1.tw, pred);
-    SCXR_LAUNCHED("fft r2c rows (wave)");
-    hipLaunchKernelGGL(w1k_cols_kernel, dim3(static_cast<unsigned>(p.batch * H1)), dim3(64), 0, s,
-                       half, p.hs, p.batch * H1, H1, static_cast<int>(adj), p.ax[0].tw, pred);
-    SCXR_LAUNCHED("fft half-spectrum columns (wave)");
-    hipLaunchKernelGGL(w1k_c2r_kernel, dim3(static_cast<unsigned>(cl)), dim3(64), 0, s, half, y,
+    SCXR_LAUNCHED("fft r2c rows (1024)");
+    hipLaunchKernelGGL(q1k_cols_kernel, dim3(static_cast<unsigned>(p.batch * H1)), dim3(QT), 0, s,
+                       half, p.hs, H1, static_cast<int>(adj), p.ax[0].tw, pred);
+    SCXR_LAUNCHED("fft half-spectrum columns (1024)");
+    hipLaunchKernelGGL(q1k_c2r_kernel, dim3(static_cast<unsigned>(cl)), dim3(QT), 0, s, half, y,
                        R, t1.tw, pred);
-    SCXR_LAUNCHED("fft c2r rows (wave)");
+    SCXR_LAUNCHED("fft c2r rows (1024)");
     return SCXR_OK;
   }
   int lr = p.lpb_rows;
@@ -1067,6 +1148,30 @@ int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int3
                      const int32_t* pred, void* stream) {
   SCXR_REQUIRE(plan && x && y, SCXR_ERR_ARG, "null argument to cconv_apply");
   return convolve(plan->p, x, y, adjoint != 0, pred, as_stream(stream));
+}
+
+int scxr_cconv_apgm_grad(const scxr_cconv_plan* plan, const float* x, const float* gs,
+                         const float* Yh, double* obj, float* gx, const float* v,
+                         const float* aty, float c, float b, float* u, const int32_t* pred,
+                         void* stream) {
+  SCXR_REQUIRE(plan && x && gs && gx && v && aty && u && (!obj || Yh), SCXR_ERR_ARG,
+               "null argument to cconv_apgm_grad");
+  const Plan& p = plan->p;
+  SCXR_REQUIRE(p.r2c && p.batch == 1 && p.F[0] == Q1K && p.F[1] == Q1K, SCXR_ERR_UNSUPPORTED,
+               "fused APGM gradient needs a single 1024 x 1024 real convolution plan");
+  const hipStream_t s = as_stream(stream);
+  constexpr int H1 = Q1K / 2 + 1;
+  float2* half = p.work;
+  hipLaunchKernelGGL(q1k_r2c_kernel, dim3(Q1K / 2), dim3(QT), 0, s, x, half,
+                     static_cast<long long>(Q1K), p.ax[1].tw, pred);
+  SCXR_LAUNCHED("fft r2c rows (1024)");
+  hipLaunchKernelGGL(q1k_cols_gram_kernel, dim3(H1), dim3(QT), 0, s, half, p.hs, gs,
+                     reinterpret_cast<const float2*>(Yh), obj, H1, p.ax[0].tw, pred);
+  SCXR_LAUNCHED("fft gram columns (1024)");
+  hipLaunchKernelGGL(q1k_c2r_step_kernel, dim3(Q1K / 2), dim3(QT), 0, s, half, gx, v, aty,
+                     1.0f + c, c, b, u, p.ax[1].tw, pred);
+  SCXR_LAUNCHED("fft c2r gradient step (1024)");
+  return SCXR_OK;
 }
 
 int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,

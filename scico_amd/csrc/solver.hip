@@ -585,6 +585,27 @@ __global__ __launch_bounds__(256) void admm_itstat_kernel(
                    counter, hist_cap);
 }
 
+// One AcceleratedPGM statistics row: hist[row] = {obj_scale * sum(obj) +
+// lam * sum(tv), sqrt(sum(res))} (Objective f + g and Residual,
+// scico/optimize/_pgm.py:151-156, 242-248), row from the device counter.
+__global__ __launch_bounds__(256) void apgm_itstat_kernel(
+    const double* __restrict__ obj, int nobj, double obj_scale, const double* __restrict__ tv,
+    int ntv, double lam, const double* __restrict__ res, int nres, double* __restrict__ hist,
+    int32_t* __restrict__ counter, int hist_cap) {
+  __shared__ double sh[4];
+  const double so = all_sum_f64(obj, nobj, sh);
+  const double st = all_sum_f64(tv, ntv, sh);
+  const double sr = all_sum_f64(res, nres, sh);
+  if (threadIdx.x == 0) {
+    const int row = counter[0];
+    if (row < hist_cap) {
+      hist[2 * row] = obj_scale * so + lam * st;
+      hist[2 * row + 1] = sqrt(sr);
+    }
+    counter[0] = row + 1;
+  }
+}
+
 // ---------------------------------------------------------------- TV --
 
 // Approximate TV prox of scico/functional/_tvnorm.py:183-236, all axes of
@@ -694,12 +715,21 @@ __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ 
 // of once per neighbouring output, and every output combines the
 // coefficients at p, p - e0 and p - e1 with the same operations, in the
 // same order, as tv_prox_kernel (so the results are bitwise equal).
+//
+// APGM: the AcceleratedPGM step that follows the prox (scico/optimize/
+// _pgm.py:221-240) fused into the store: with out holding x_prev and vm the
+// previous extrapolated point v, each output x also gives the partial sum
+// of (x - v)^2 (the fixed-point residual, float64, one per workgroup) and
+// v <- x + c (x - x_prev), with the operations of the unfused path.
 constexpr int TVT = 16;
+template <bool APGM>
 __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict__ v,
                                                         float* __restrict__ out, TvDims t,
                                                         float tau, float invK, int iso,
                                                         const float* __restrict__ w,
-                                                        const int32_t* pred) {
+                                                        const int32_t* pred,
+                                                        float* __restrict__ vm, float c,
+                                                        double* __restrict__ res_ws) {
   if (pred_off(pred)) return;
   __shared__ float vs[TVT + 2][TVT + 3];
   __shared__ float cH[2][TVT + 1][TVT + 2], cL[2][TVT + 1][TVT + 2];
@@ -737,6 +767,7 @@ __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict_
   __syncthreads();
   const int ty = tid / TVT, tx = tid - ty * TVT;
   const int pi = i0 + ty, pj = j0 + tx;
+  double res = 0.0;
   if (pi < M && pj < N) {
     // p at (ty + 1, tx + 1); p - e0 at (ty, tx + 1); p - e1 at (ty + 1, tx)
     const float lo0 = __fmul_rn(s, __fadd_rn(cL[0][ty + 1][tx + 1], cL[0][ty][tx + 1]));
@@ -744,7 +775,20 @@ __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict_
     const float lo1 = __fmul_rn(s, __fadd_rn(cL[1][ty + 1][tx + 1], cL[1][ty + 1][tx]));
     const float hi1 = __fmul_rn(s, __fsub_rn(cH[1][ty + 1][tx], cH[1][ty + 1][tx + 1]));
     const float acc = __fadd_rn(__fadd_rn(lo0, hi0), __fadd_rn(lo1, hi1));
-    out[static_cast<int64_t>(pi) * N + pj] = __fmul_rn(invK, acc);
+    const float xn = __fmul_rn(invK, acc);
+    const int64_t e = static_cast<int64_t>(pi) * N + pj;
+    if constexpr (APGM) {
+      const float xo = out[e], vo = vm[e];
+      const double d = static_cast<double>(xn) - static_cast<double>(vo);
+      res = d * d;
+      vm[e] = __fadd_rn(xn, __fmul_rn(c, __fsub_rn(xn, xo)));
+    }
+    out[e] = xn;
+  }
+  if constexpr (APGM) {
+    __shared__ double sh[4];
+    res = block_sum_f64(res, sh);
+    if (tid == 0) res_ws[blockIdx.y * gridDim.x + blockIdx.x] = res;
   }
 }
 
@@ -1233,6 +1277,18 @@ int scxr_admm_itstat(const double* res, int32_t nres, const double* g, int32_t n
 
 int32_t scxr_elem_blocks(int64_t n) { return static_cast<int32_t>(solver::elem_grid(n)); }
 
+int scxr_apgm_itstat(const double* obj, int32_t nobj, double obj_scale, const double* tv,
+                     int32_t ntv, double lam, const double* res, int32_t nres, double* hist,
+                     int32_t* counter, int32_t hist_cap, void* stream) {
+  SCXR_REQUIRE(obj && tv && res && hist && counter && nobj > 0 && ntv > 0 && nres > 0 &&
+                   hist_cap > 0,
+               SCXR_ERR_ARG, "bad apgm_itstat arguments");
+  hipLaunchKernelGGL(solver::apgm_itstat_kernel, dim3(1), dim3(256), 0, as_stream(stream), obj,
+                     nobj, obj_scale, tv, ntv, lam, res, nres, hist, counter, hist_cap);
+  SCXR_LAUNCHED("apgm itstat");
+  return SCXR_OK;
+}
+
 static int tv_dims(int32_t ndim, const int32_t* shape, int32_t circular, solver::TvDims& t) {
   SCXR_REQUIRE(ndim >= 1 && ndim <= 3 && shape, SCXR_ERR_ARG, "bad TV shape");
   for (int a = 0; a < 3; ++a) t.d[a] = 1;
@@ -1255,14 +1311,35 @@ int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out,
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
   if (ndim == 2) {
     const dim3 g(ceil_div(t.d[2], solver::TVT), ceil_div(t.d[1], solver::TVT));
-    hipLaunchKernelGGL(solver::tv_prox2d_kernel, g, dim3(256), 0, as_stream(stream), v, out, t,
-                       tau, 0.25f, isotropic, nullptr, pred);
+    hipLaunchKernelGGL(solver::tv_prox2d_kernel<false>, g, dim3(256), 0, as_stream(stream), v,
+                       out, t, tau, 0.25f, isotropic, nullptr, pred, nullptr, 0.0f, nullptr);
   } else {
     hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                        as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
                        isotropic, nullptr, pred);
   }
   SCXR_LAUNCHED("tv prox");
+  return SCXR_OK;
+}
+
+int32_t scxr_tv_prox_apgm_parts(int32_t M, int32_t N) {
+  if (M <= 0 || N <= 0) return -1;
+  return static_cast<int32_t>(ceil_div(M, solver::TVT) * ceil_div(N, solver::TVT));
+}
+
+int scxr_tv_prox_apgm(int32_t M, int32_t N, const float* u, float* x, float* v, float tau,
+                      int32_t isotropic, int32_t circular, float c, double* res_ws,
+                      const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(u && x && v && res_ws && u != x && u != v && x != v, SCXR_ERR_ARG,
+               "bad tv_prox_apgm arguments (u, x and v must be distinct)");
+  const int32_t shape[2] = {M, N};
+  solver::TvDims t;
+  const int rc = tv_dims(2, shape, circular, t);
+  if (rc) return rc;
+  const dim3 g(ceil_div(N, solver::TVT), ceil_div(M, solver::TVT));
+  hipLaunchKernelGGL(solver::tv_prox2d_kernel<true>, g, dim3(256), 0, as_stream(stream), u, x, t,
+                     tau, 0.25f, isotropic, nullptr, pred, v, c, res_ws);
+  SCXR_LAUNCHED("tv prox apgm");
   return SCXR_OK;
 }
 
@@ -1277,8 +1354,8 @@ int scxr_tv_prox_w(int32_t ndim, const int32_t* shape, const float* v, const flo
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
   if (ndim == 2) {
     const dim3 g(ceil_div(t.d[2], solver::TVT), ceil_div(t.d[1], solver::TVT));
-    hipLaunchKernelGGL(solver::tv_prox2d_kernel, g, dim3(256), 0, as_stream(stream), v, out, t,
-                       tau, 0.25f, isotropic, w, pred);
+    hipLaunchKernelGGL(solver::tv_prox2d_kernel<false>, g, dim3(256), 0, as_stream(stream), v,
+                       out, t, tau, 0.25f, isotropic, w, pred, nullptr, 0.0f, nullptr);
   } else {
     hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                        as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
@@ -1313,6 +1390,19 @@ int scxr_tv_prox_vjp(int32_t ndim, const int32_t* shape, const float* v, const f
   hipLaunchKernelGGL(solver::tv_vjp_v_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                      as_stream(stream), HB, LB, t, gv);
   SCXR_LAUNCHED("tv prox vjp v");
+  return SCXR_OK;
+}
+
+int scxr_tv_norm_partials(int32_t ndim, const int32_t* shape, const float* x, int32_t isotropic,
+                          int32_t circular, double* ws, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(x && ws, SCXR_ERR_ARG, "bad tv_norm_partials arguments");
+  solver::TvDims t;
+  const int rc = tv_dims(ndim, shape, circular, t);
+  if (rc) return rc;
+  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
+  hipLaunchKernelGGL(solver::tv_norm_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), x, t, isotropic, ws, pred);
+  SCXR_LAUNCHED("tv norm partials");
   return SCXR_OK;
 }
 

@@ -264,3 +264,128 @@ def test_squared_l2_grad_circconv(gpu, kind):
     f.A = B
     refB = B.adj(B(x) - y)
     assert rel_l2(f.grad(x).cpu().numpy(), refB.cpu().numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_apgm_deconvolution_config_length(gpu, fused):
+    """The GPU-scale configuration's deconvolution at full length: 200
+    AcceleratedPGM-TV iterations at 1024^2 (15 x 15 Gaussian PSF, 1 % noise,
+    lam 2e-2) against the oracle's run (``tests/golden/apgm_deconv_1024.npz``,
+    written by ``tests/golden/make_deconv_fixture.py``).  APGM does not
+    amplify rounding the way ten-step CG on the CT normal equations does:
+    the other valid float32 runs (float64 FFTs, FFT-rounding-sized noise)
+    end 1.4e-7 (rel-L2) from the oracle, so the reconstruction is held to
+    the north star's 1e-5 directly; the objective and residual statistics
+    to 1e-5 of their value (residual: of its largest value so far).  Both
+    the fused iteration (one convolution per iteration: the gradient by
+    linearity, the objective by Parseval) and the generic one."""
+    import json
+    import os
+    import sys
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    if gold not in sys.path:
+        sys.path.insert(0, gold)
+    import make_deconv_fixture as D
+
+    fx = np.load(os.path.join(gold, "apgm_deconv_1024.npz"), allow_pickle=False)
+    meta = json.loads(str(fx["meta"]))
+    xgt, psf, hd, y, _ = D.problem()
+    assert D.sha256(y) == meta["y_sha256"], "regenerated blurred image differs from the fixture's"
+    N = D.N
+    A = CircularConvolve(psf, (N, N), h_center=[7, 7])
+    f = loss.SquaredL2Loss(y=dev(y), A=A)
+    g = D.LAM * functional.IsotropicTVNorm()
+    from scico_amd.optimize import _pgm
+
+    old = _pgm._FusedDeconv.enabled
+    _pgm._FusedDeconv.enabled = fused
+    try:
+        solver = AcceleratedPGM(f=f, g=g, L0=meta["L0"], x0=dev(np.zeros((N, N), np.float32)),
+                                maxiter=D.ITERS)
+    finally:
+        _pgm._FusedDeconv.enabled = old
+    assert (solver._fused is not None) == fused
+    x = solver.solve().cpu().numpy()
+    h = solver.history(transpose=True)
+    hr = fx["hist"]
+    d_obj = np.abs(np.asarray(h.Objective, np.float64) - hr[:, 0]) / np.abs(hr[:, 0])
+    d_res = np.abs(np.asarray(h.Residual, np.float64) - hr[:, 1]) / \
+        np.maximum.accumulate(np.abs(hr[:, 1]))
+    e_crop, e_pool = rel_l2(D.crop(x), fx["x_crop"]), rel_l2(D.pool(x), fx["x_pool"])
+    snr = D.snr(x, xgt)
+    fin = meta["floor_final"]
+    print(f"  APGM-TV deconvolution 1024^2 x{D.ITERS} ({'fused' if fused else 'generic'}): central crop rel-L2 {e_crop:.2e} (floor "
+          f"{fin['x_crop_rel_l2']:.2e}), pooled {e_pool:.2e} (floor {fin['x_pool_rel_l2']:.2e}); "
+          f"objective dev max {d_obj.max():.2e}, residual dev max {d_res.max():.2e}; SNR "
+          f"{snr:.4f} vs {meta['snr_db']:.4f} dB", flush=True)
+    assert e_crop <= 1e-5 and e_pool <= 1e-5
+    assert d_obj.max() <= 1e-5 and d_res.max() <= 1e-5
+    assert abs(snr - meta["snr_db"]) <= 1e-3
+
+
+def _deconv_1024(fused, maxiter, x0=None):
+    from scico_amd.optimize import _pgm
+
+    xgt, h, hd, y, L0 = _deconv_problem(1024)
+    A = CircularConvolve(h, (1024, 1024), h_center=[7, 7])
+    f = loss.SquaredL2Loss(y=dev(y), A=A)
+    g = 2e-2 * functional.AnisotropicTVNorm()
+    old = _pgm._FusedDeconv.enabled
+    _pgm._FusedDeconv.enabled = fused
+    try:
+        x0 = dev(np.zeros((1024, 1024), np.float32)) if x0 is None else x0
+        s = AcceleratedPGM(f=f, g=g, L0=L0, x0=x0, maxiter=maxiter)
+    finally:
+        _pgm._FusedDeconv.enabled = old
+    assert (s._fused is not None) == fused
+    return s
+
+
+def test_fused_apgm_first_step_and_state(gpu):
+    """The fused deconvolution iteration (anisotropic TV here): its first
+    step is the generic one bit for bit (the extrapolation coefficient is 0
+    there, so G v is the convolution itself), the statistics agree to
+    float32 resolution; after 30 steps the iterates agree to 1e-6; a state
+    saved after 10 steps and restored into a new solver continues to the
+    same iterate (1e-6); 4 steps captured in a HIP graph and replayed equal
+    4 eager steps bit for bit."""
+    import tempfile
+
+    sf, sg = _deconv_1024(True, 1), _deconv_1024(False, 1)
+    sf.solve()
+    sg.solve()
+    assert torch.equal(sf.x, sg.x) and torch.equal(sf.v, sg.v)
+    hf, hg = sf.history(transpose=True), sg.history(transpose=True)
+    assert abs(float(hf.Residual[0]) - float(hg.Residual[0])) <= 1e-12 * float(hg.Residual[0])
+    assert abs(float(hf.Objective[0]) - float(hg.Objective[0])) <= 1e-6 * float(hg.Objective[0])
+    sf, sg = _deconv_1024(True, 30), _deconv_1024(False, 30)
+    xf, xg = sf.solve().cpu().numpy(), sg.solve().cpu().numpy()
+    e30 = rel_l2(xf, xg)
+    # save after 10, restore, continue 20
+    s1 = _deconv_1024(True, 10)
+    s1.solve()
+    with tempfile.TemporaryDirectory() as d:
+        path = d + "/state.npz"
+        s1.save_state(path)
+        s2 = _deconv_1024(True, 20)
+        s2.load_state(path)
+    s2.t = s1.t
+    xr = s2.solve().cpu().numpy()
+    e_restore = rel_l2(xr, xf)
+    # graph capture of 4 steps against 4 eager steps
+    se, sc = _deconv_1024(True, 4), _deconv_1024(True, 4)
+    for _ in range(4):
+        se.step()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(4):
+            sc.step()
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    same = torch.equal(sc.x, se.x) and torch.equal(sc._fused.hist[:4], se._fused.hist[:4])
+    print(f"  fused APGM 1024^2: 30 steps vs generic rel-L2 {e30:.2e}; restored run "
+          f"{e_restore:.2e}; graph replay bitwise {same}", flush=True)
+    assert e30 <= 1e-6 and e_restore <= 1e-6 and same
