@@ -45,6 +45,8 @@ def load(lib: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib_.cpu_xr2d_adjoint.argtypes = [i, i, i, i, P, P, d, P, P, P]
     lib_.cpu_xr2d_forward_f64acc.argtypes = [i, i, i, i, P, P, d, P, P, P]
     lib_.cpu_xr2d_adjoint_f64acc.argtypes = [i, i, i, i, P, P, d, P, P, P]
+    lib_.cpu_xr3d_sep_forward.argtypes = [i, i, i, i, i, i, P, i, i, i, P, P]
+    lib_.cpu_xr3d_sep_adjoint.argtypes = [i, i, i, i, i, i, P, i, i, i, P, P]
     return lib_
 
 

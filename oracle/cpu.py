@@ -97,3 +97,49 @@ def admm_tv(y, g, lam, rho, x0, maxiter, cg_maxiter=10, cg_tol=1e-4):
     """ADMM-TV (oracle.solvers.admm_tv) with the C/OpenMP projector."""
     return S.admm_tv(y, lambda v: project(v, g), lambda v: back_project(v, g), lam, rho, x0,
                      maxiter, cg_tol=cg_tol, cg_maxiter=cg_maxiter)
+
+
+def xr3d_separable(matrices):
+    """(s, ri) if the view matrices (V, 2, 4) rotate about volume axis s:
+    detector row ri has coefficient 0 on axis s in every view and row
+    1 - ri coefficient 0 on the two other axes; else None."""
+    m = np.asarray(matrices, dtype=np.float32)
+    for s in range(3):
+        others = [a for a in range(3) if a != s]
+        for ri in range(2):
+            rs = 1 - ri
+            if np.all(m[:, ri, s] == 0) and np.all(m[:, rs, others] == 0):
+                return s, ri
+    return None
+
+
+def _sep_call(fn, data, matrices, vol_shape, det_shape, f64acc):
+    sep = xr3d_separable(matrices)
+    if sep is None:
+        raise ValueError("view matrices are not separable (no common rotation axis)")
+    m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float32))
+    X, Y, Z = vol_shape
+    D0, D1 = det_shape
+    return fn(X, Y, Z, len(m), D0, D1, m.ctypes.data, sep[0], sep[1], int(f64acc), data)
+
+
+def project_3d_sep(vol, matrices, det_shape, f64acc=False):
+    """``oracle.xray.project_3d`` for a separable geometry (C/OpenMP)."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    out = np.empty((len(matrices), *det_shape), np.float32)
+    rc = _sep_call(lambda *a: lib().cpu_xr3d_sep_forward(*a, out.ctypes.data), vol.ctypes.data,
+                   matrices, vol.shape, det_shape, f64acc)
+    if rc:
+        raise MemoryError("cpu_xr3d_sep_forward failed")
+    return out
+
+
+def back_project_3d_sep(proj, matrices, input_shape, f64acc=False):
+    """``oracle.xray.back_project_3d`` for a separable geometry (C/OpenMP)."""
+    proj = np.ascontiguousarray(proj, dtype=np.float32)
+    out = np.empty(tuple(input_shape), np.float32)
+    rc = _sep_call(lambda *a: lib().cpu_xr3d_sep_adjoint(*a, out.ctypes.data), proj.ctypes.data,
+                   matrices, input_shape, proj.shape[1:], f64acc)
+    if rc:
+        raise MemoryError("cpu_xr3d_sep_adjoint failed")
+    return out
