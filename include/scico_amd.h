@@ -287,21 +287,14 @@ int scxr_tv_prox_vjp(int32_t ndim, const int32_t* shape /* host */, const float*
 int scxr_tv_norm(int32_t ndim, const int32_t* shape, const float* x, int32_t isotropic,
                  int32_t circular, double* out, double* ws, const int32_t* pred, void* stream);
 
-/* The per-workgroup float64 partials of scxr_tv_norm (scxr_elem_blocks(n)
- * of them) without the final sum. */
-int scxr_tv_norm_partials(int32_t ndim, const int32_t* shape /* host */, const float* x,
-                          int32_t isotropic, int32_t circular, double* ws,
-                          const int32_t* pred, void* stream);
-
 /* ------------------------------- fused AcceleratedPGM deconvolution --
  * One AcceleratedPGM-TV iteration (scico/optimize/_pgm.py:183-252) on a
  * 1024 x 1024 image with f = 0.5 ||A x - y||^2, A a real CircularConvolve,
- * g = lam TV (2D), constant step 1/L, in four launches and one statistics
+ * g = lam TV (2D), constant step 1/L, in five launches and one statistics
  * row:
  *   scxr_tv_prox_apgm:  x <- prox_{tau TV}(u) (tau = lam / L), the
- *     residual partials of ||x - v||, v <- x + c (x - x_prev)
+ *     partials of ||x - v||^2 and of TV(x), v <- x + c (x - x_prev)
  *     (c = (t_k - 1) / t_{k+1});
- *   scxr_tv_norm_partials on x;
  *   scxr_cconv_apgm_grad: G x and the next gradient step u (see there),
  *     with the objective's partials;
  *   scxr_apgm_itstat: the Objective and Residual row.
@@ -309,11 +302,12 @@ int scxr_tv_norm_partials(int32_t ndim, const int32_t* shape /* host */, const f
 /* Workgroup (partial) count of scxr_tv_prox_apgm for an M x N image. */
 int32_t scxr_tv_prox_apgm_parts(int32_t M, int32_t N);
 /* x <- prox of u as scxr_tv_prox (2D); res_ws[b] = sum over workgroup b of
- * (x - v)^2 (float64); v <- x + c (x - x_prev) with x_prev the x on entry.
- * u, x and v are distinct M x N buffers. */
+ * (x - v)^2 and tv_ws[b] its share of the TV norm of the new x (as
+ * scxr_tv_norm; both float64); v <- x + c (x - x_prev) with x_prev the x
+ * on entry.  u, x and v are distinct M x N buffers. */
 int scxr_tv_prox_apgm(int32_t M, int32_t N, const float* u, float* x, float* v, float tau,
                       int32_t isotropic, int32_t circular, float c, double* res_ws,
-                      const int32_t* pred, void* stream);
+                      double* tv_ws, const int32_t* pred, void* stream);
 /* hist[2 row] = obj_scale * sum(obj) + lam * sum(tv), hist[2 row + 1] =
  * sqrt(sum(res)), row = *counter (skipped at or beyond hist_cap), then
  * *counter += 1. */
@@ -348,14 +342,15 @@ int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int3
                      const int32_t* pred, void* stream);
 /* Gradient step of the fused AcceleratedPGM iteration, for a plan of one
  * 1024 x 1024 real convolution (SCXR_ERR_UNSUPPORTED otherwise); G = A^T A
- * is the convolution by gs = |Hs|^2 (float32, 1024 x 513, Hs the Hermitian
- * part of the spectrum on the half grid):
+ * is the convolution by gs = |Hs|^2 (Hs the Hermitian part of the spectrum
+ * on the half grid k0 < 1024, k1 < 513; gs float32 and Yh complex64 are
+ * stored transposed, gs[k1 1024 + k0]):
  *   gx <- G x;  u <- v + b ((1 + c) gx - c gx_prev - aty)
  * with gx_prev the gx on entry (G v for v = x + c (x - x_prev), by
  * linearity; c = 0 at the first step, with gx zero), aty = A^T y and
  * b = -1/L.  If obj is non-NULL, obj[k] (513 doubles) = w_k sum over the
- * column k of |Hs X - Yh|^2 (X = fft2(x), Yh = fft2(y) on the half grid,
- * complex64; w_k = 1 for k = 0 and 512, else 2), so that sum(obj) / 1024^2
+ * column k of |Hs X - Yh|^2 (X = fft2(x), Yh = fft2(y) on the half grid;
+ * w_k = 1 for k = 0 and 512, else 2), so that sum(obj) / 1024^2
  * = ||A x - y||^2 (Parseval).  No reference counterpart as one call: it
  * is the gradient of SquaredL2Loss (scico/loss.py:208-209) and the
  * objective evaluation of _pgm.py:151-156, one convolution instead of

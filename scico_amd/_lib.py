@@ -145,13 +145,11 @@ SIGNATURES = [
       c_void_p, c_void_p, c_void_p]),
     ("scxr_tv_norm", c_int32,
      [c_int32, c_int32_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
-    ("scxr_tv_norm_partials", c_int32,
-     [c_int32, c_int32_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
     # fused AcceleratedPGM deconvolution
     ("scxr_tv_prox_apgm_parts", c_int32, [c_int32, c_int32]),
     ("scxr_tv_prox_apgm", c_int32,
      [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_float,
-      c_void_p, c_void_p, c_void_p]),
+      c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_apgm_itstat", c_int32,
      [c_void_p, c_int32, c_double, c_void_p, c_int32, c_double, c_void_p, c_int32, c_void_p,
       c_void_p, c_int32, c_void_p]),

@@ -557,8 +557,7 @@ struct QTw {
   float2 w[4][3];  // stages 1..4, r = 1..3
 };
 
-__device__ __forceinline__ void q_twiddles(QTw& t, const float2* __restrict__ tw) {
-  const int j = threadIdx.x;
+__device__ __forceinline__ void q_twiddles(QTw& t, const float2* __restrict__ tw, int j) {
 #pragma unroll
   for (int s = 1; s < 5; ++s) {
     const int Ns = 1 << (2 * s);
@@ -568,9 +567,10 @@ __device__ __forceinline__ void q_twiddles(QTw& t, const float2* __restrict__ tw
   }
 }
 
+// Transform of thread j's four values on the line buffer lds (every thread
+// of the workgroup calls it: the barriers are workgroup wide).
 template <bool INV>
-__device__ __forceinline__ void q_fft(float2 (&a)[4], float2* lds, const QTw& t) {
-  const int j = threadIdx.x;
+__device__ __forceinline__ void q_fft(float2 (&a)[4], float2* lds, const QTw& t, int j) {
 #pragma unroll
   for (int s = 0; s < 5; ++s) {
     const int Ns = 1 << (2 * s);
@@ -592,110 +592,133 @@ __device__ __forceinline__ void q_fft(float2 (&a)[4], float2* lds, const QTw& t)
   }
 }
 
-// Forward rows: two real rows per complex transform -> half spectra (R rows
-// of 513).  z = x_a + i x_b; X_a[k] = (Z[k] + conj Z[-k]) / 2,
-// X_b[k] = (Z[k] - conj Z[-k]) / 2i.
-__global__ __launch_bounds__(QT) void q1k_r2c_kernel(const float* __restrict__ x,
-                                                     float2* __restrict__ half, long long R,
-                                                     const float2* __restrict__ tw,
-                                                     const int32_t* pred) {
+// The 1024 x 1024 half spectrum is kept transposed, column k1 of every item
+// contiguous: halfT[(item H1 + k1) 1024 + row], so that the column
+// transforms read and write whole lines.  The row kernels take QR = 4 rows
+// (two complex transforms, 512 threads) per workgroup and move each
+// column's four values of those rows as two 16-byte pairs.
+constexpr int QR = 4;
+constexpr int QH1 = Q1K / 2 + 1;
+
+// Row block of a row-kernel workgroup: consecutive workgroups go to the 8
+// XCDs in turn, so each XCD takes a run of neighbouring row blocks, whose
+// pieces of one transposed column share cache lines (a bijection of the
+// first 8 * floor(G / 8) workgroups; the rest keep their own block).
+__device__ __forceinline__ long long q_row_block() {
+  long long b = blockIdx.x;
+  const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
+  if (b < G8) b = (b % 8) * (G8 / 8) + b / 8;
+  return b;
+}
+
+// Forward rows: two real rows per complex transform (z = x_a + i x_b;
+// X_a[k] = (Z[k] + conj Z[-k]) / 2, X_b[k] = (Z[k] - conj Z[-k]) / 2i).
+__global__ __launch_bounds__(2 * QT) void q1k_r2c_kernel(const float* __restrict__ x,
+                                                         float2* __restrict__ halfT,
+                                                         const float2* __restrict__ tw,
+                                                         const int32_t* pred) {
   if (pred_off(pred)) return;
-  __shared__ float2 lds[Q_LDS];
-  const int j = threadIdx.x;
-  const long long ra = 2ll * blockIdx.x, rb = ra + 1;
+  __shared__ float2 lds[2][Q_LDS];
+  const int g = threadIdx.x >> 8, j = threadIdx.x & (QT - 1);
+  const long long r0 = q_row_block() * QR;  // first row (all in one item)
+  const long long ra = r0 + 2 * g;
   QTw t;
-  q_twiddles(t, tw);
+  q_twiddles(t, tw, j);
   float2 a[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    a[r] = make_float2(x[ra * Q1K + j + QT * r], rb < R ? x[rb * Q1K + j + QT * r] : 0.0f);
-  q_fft<false>(a, lds, t);
+    a[r] = make_float2(x[ra * Q1K + j + QT * r], x[(ra + 1) * Q1K + j + QT * r]);
+  q_fft<false>(a, lds[g], t, j);
   __syncthreads();  // stage 4's reads are done
 #pragma unroll
-  for (int r = 0; r < 4; ++r) lds[q_pad(j + QT * r)] = a[r];
+  for (int r = 0; r < 4; ++r) lds[g][q_pad(j + QT * r)] = a[r];
   __syncthreads();
-  constexpr int H1 = Q1K / 2 + 1;
-  for (int k = j; k < H1; k += QT) {
-    const float2 zk = lds[q_pad(k)], zm = lds[q_pad((Q1K - k) & (Q1K - 1))];
-    half[ra * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.x, zm.x)),
-                                    __fmul_rn(0.5f, __fsub_rn(zk.y, zm.y)));
-    if (rb < R)
-      half[rb * H1 + k] = make_float2(__fmul_rn(0.5f, __fadd_rn(zk.y, zm.y)),
-                                      __fmul_rn(0.5f, __fsub_rn(zm.x, zk.x)));
+  const long long item = r0 / Q1K;
+  const int rl = static_cast<int>(r0 - item * Q1K);
+  float4* out = reinterpret_cast<float4*>(halfT + item * QH1 * Q1K + rl);
+  for (int e = threadIdx.x; e < 2 * QH1; e += 2 * QT) {
+    const int k = e >> 1, h = e & 1;
+    const float2 zk = lds[h][q_pad(k)], zm = lds[h][q_pad((Q1K - k) & (Q1K - 1))];
+    out[(static_cast<long long>(k) * Q1K >> 1) + h] =
+        make_float4(__fmul_rn(0.5f, __fadd_rn(zk.x, zm.x)), __fmul_rn(0.5f, __fsub_rn(zk.y, zm.y)),
+                    __fmul_rn(0.5f, __fadd_rn(zk.y, zm.y)), __fmul_rn(0.5f, __fsub_rn(zm.x, zk.x)));
   }
 }
 
-// Columns of the half spectrum (F0 = 1024): forward, times hs (conj for
-// the adjoint), inverse, times 1/1024; one column per workgroup.
-__global__ __launch_bounds__(QT) void q1k_cols_kernel(float2* __restrict__ half,
-                                                      const float2* __restrict__ hs, int H1,
-                                                      int conj_h, const float2* __restrict__ tw,
+// Columns (F0 = 1024) of the transposed half spectrum: forward, times hsT
+// (conj for the adjoint), inverse, times 1/1024; one column per workgroup.
+__global__ __launch_bounds__(QT) void q1k_cols_kernel(float2* __restrict__ halfT,
+                                                      const float2* __restrict__ hsT, int conj_h,
+                                                      const float2* __restrict__ tw,
                                                       const int32_t* pred) {
   if (pred_off(pred)) return;
   __shared__ float2 lds[Q_LDS];
   const int j = threadIdx.x;
-  // consecutive workgroups go to the 8 XCDs in turn: give each XCD a run
-  // of neighbouring columns, which share cache lines (a bijection of the
-  // first 8 * floor(G / 8) workgroups; the rest keep their own column)
-  long long L = blockIdx.x;
-  {
-    const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
-    if (L < G8) L = (L % 8) * (G8 / 8) + L / 8;
-  }
-  const long long item = L / H1;
-  const int c = static_cast<int>(L - item * H1);
-  float2* col = half + item * static_cast<long long>(Q1K) * H1 + c;
+  const int c = static_cast<int>(blockIdx.x % QH1);
+  float2* col = halfT + static_cast<long long>(blockIdx.x) * Q1K;
+  const float2* hc = hsT + static_cast<long long>(c) * Q1K;
   QTw t;
-  q_twiddles(t, tw);
+  q_twiddles(t, tw, j);
   float2 h[4], a[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const float2 hv = hs[static_cast<long long>(j + QT * r) * H1 + c];
+    const float2 hv = hc[j + QT * r];
     h[r] = conj_h ? conjf2(hv) : hv;
+    a[r] = col[j + QT * r];
   }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) a[r] = col[static_cast<long long>(j + QT * r) * H1];
-  q_fft<false>(a, lds, t);
+  q_fft<false>(a, lds, t, j);
 #pragma unroll
   for (int r = 0; r < 4; ++r) a[r] = cmul(h[r], a[r]);
-  q_fft<true>(a, lds, t);
+  q_fft<true>(a, lds, t, j);
   constexpr float sc = 1.0f / Q1K;
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    col[static_cast<long long>(j + QT * r) * H1] =
-        make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
+    col[j + QT * r] = make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
 }
 
-// Inverse rows: half spectra -> two real rows per transform, times 1/1024.
-__global__ __launch_bounds__(QT) void q1k_c2r_kernel(const float2* __restrict__ half,
-                                                     float* __restrict__ y, long long R,
-                                                     const float2* __restrict__ tw,
-                                                     const int32_t* pred) {
-  if (pred_off(pred)) return;
-  __shared__ float2 lds[Q_LDS];
-  const int j = threadIdx.x;
-  const long long ra = 2ll * blockIdx.x, rb = ra + 1;
-  QTw t;
-  q_twiddles(t, tw);
-  constexpr int H1 = Q1K / 2 + 1, nh = Q1K / 2;
-  for (int k = j; k < H1; k += QT) {
-    float2 a = half[ra * H1 + k];
-    float2 b = rb < R ? half[rb * H1 + k] : make_float2(0.0f, 0.0f);
-    if (k == 0 || k == nh) a.y = b.y = 0.0f;
-    lds[q_pad(k)] = make_float2(__fsub_rn(a.x, b.y), __fadd_rn(a.y, b.x));
+// Inverse rows: the transposed half spectra of QR rows -> real rows, times
+// 1/1024 (Z[k] = X_a[k] + i X_b[k] and its Hermitian extension, imaginary
+// parts of k = 0 and 512 dropped as irfft does).  Loads the transform
+// result of thread j of group g into a (pair of rows 2g, 2g + 1).
+__device__ __forceinline__ void q1k_c2r_load(const float2* __restrict__ halfT, float2 (&lds)[2][Q_LDS],
+                                             const QTw& t, int g, int j, float2 (&a)[4]) {
+  const long long r0 = q_row_block() * QR;
+  const long long item = r0 / Q1K;
+  const int rl = static_cast<int>(r0 - item * Q1K);
+  const float4* in = reinterpret_cast<const float4*>(halfT + item * QH1 * Q1K + rl);
+  constexpr int nh = Q1K / 2;
+  for (int e = threadIdx.x; e < 2 * QH1; e += 2 * QT) {
+    const int k = e >> 1, h = e & 1;
+    const float4 v = in[(static_cast<long long>(k) * Q1K >> 1) + h];
+    float2 aa = make_float2(v.x, v.y), bb = make_float2(v.z, v.w);
+    if (k == 0 || k == nh) aa.y = bb.y = 0.0f;
+    lds[h][q_pad(k)] = make_float2(__fsub_rn(aa.x, bb.y), __fadd_rn(aa.y, bb.x));
     if (k > 0 && k < nh)
-      lds[q_pad(Q1K - k)] = make_float2(__fadd_rn(a.x, b.y), __fsub_rn(b.x, a.y));
+      lds[h][q_pad(Q1K - k)] = make_float2(__fadd_rn(aa.x, bb.y), __fsub_rn(bb.x, aa.y));
   }
   __syncthreads();
-  float2 a[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) a[r] = lds[q_pad(j + QT * r)];
-  q_fft<true>(a, lds, t);
+  for (int r = 0; r < 4; ++r) a[r] = lds[g][q_pad(j + QT * r)];
+  q_fft<true>(a, lds[g], t, j);
+}
+
+__global__ __launch_bounds__(2 * QT) void q1k_c2r_kernel(const float2* __restrict__ halfT,
+                                                         float* __restrict__ y,
+                                                         const float2* __restrict__ tw,
+                                                         const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ float2 lds[2][Q_LDS];
+  const int g = threadIdx.x >> 8, j = threadIdx.x & (QT - 1);
+  QTw t;
+  q_twiddles(t, tw, j);
+  float2 a[4];
+  q1k_c2r_load(halfT, lds, t, g, j, a);
+  const long long ra = q_row_block() * QR + 2 * g;
   constexpr float sc = 1.0f / Q1K;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     y[ra * Q1K + j + QT * r] = __fmul_rn(a[r].x, sc);
-    if (rb < R) y[rb * Q1K + j + QT * r] = __fmul_rn(a[r].y, sc);
+    y[(ra + 1) * Q1K + j + QT * r] = __fmul_rn(a[r].y, sc);
   }
 }
 
@@ -710,43 +733,38 @@ __global__ __launch_bounds__(QT) void q1k_c2r_kernel(const float2* __restrict__ 
 //   * ||A x - y||^2 = sum_k w_k |Hs[k] X[k] - Y[k]|^2 / (F0 F1) over the
 //     half spectrum (Parseval; w_k = 1 on the first and the Nyquist column,
 //     2 elsewhere), X and Y the spectra of x and y.
+// All half-grid arrays are transposed (column k1 contiguous).
 
-// Columns: forward, objective partial of the column (if obj), times gs,
+// Columns: forward, objective partial of the column (if obj), times gsT,
 // inverse, times 1/1024.
-__global__ __launch_bounds__(QT) void q1k_cols_gram_kernel(float2* __restrict__ half,
-                                                           const float2* __restrict__ hs,
-                                                           const float* __restrict__ gs,
-                                                           const float2* __restrict__ Y,
-                                                           double* __restrict__ obj, int H1,
+__global__ __launch_bounds__(QT) void q1k_cols_gram_kernel(float2* __restrict__ halfT,
+                                                           const float2* __restrict__ hsT,
+                                                           const float* __restrict__ gsT,
+                                                           const float2* __restrict__ YT,
+                                                           double* __restrict__ obj,
                                                            const float2* __restrict__ tw,
                                                            const int32_t* pred) {
   if (pred_off(pred)) return;
   __shared__ float2 lds[Q_LDS];
   __shared__ double sh[4];
   const int j = threadIdx.x;
-  long long L = blockIdx.x;
-  {
-    const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
-    if (L < G8) L = (L % 8) * (G8 / 8) + L / 8;
-  }
-  const int c = static_cast<int>(L);
-  float2* col = half + c;
+  const int c = blockIdx.x;
+  const long long o = static_cast<long long>(c) * Q1K;
+  float2* col = halfT + o;
   QTw t;
-  q_twiddles(t, tw);
+  q_twiddles(t, tw, j);
   float g[4];
   float2 h[4], yk[4], a[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const long long e = static_cast<long long>(j + QT * r) * H1 + c;
-    g[r] = gs[e];
+    g[r] = gsT[o + j + QT * r];
     if (obj) {
-      h[r] = hs[e];
-      yk[r] = Y[e];
+      h[r] = hsT[o + j + QT * r];
+      yk[r] = YT[o + j + QT * r];
     }
+    a[r] = col[j + QT * r];
   }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) a[r] = col[static_cast<long long>(j + QT * r) * H1];
-  q_fft<false>(a, lds, t);
+  q_fft<false>(a, lds, t, j);
   if (obj) {
     double acc = 0.0;
 #pragma unroll
@@ -757,52 +775,39 @@ __global__ __launch_bounds__(QT) void q1k_cols_gram_kernel(float2* __restrict__ 
       acc += dx * dx + dy * dy;
     }
     acc = block_sum_f64(acc, sh);
-    if (j == 0) obj[L] = (c == 0 || c == H1 - 1) ? acc : 2.0 * acc;
+    if (j == 0) obj[c] = (c == 0 || c == QH1 - 1) ? acc : 2.0 * acc;
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) a[r] = make_float2(__fmul_rn(g[r], a[r].x), __fmul_rn(g[r], a[r].y));
-  q_fft<true>(a, lds, t);
+  q_fft<true>(a, lds, t, j);
   constexpr float sc = 1.0f / Q1K;
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    col[static_cast<long long>(j + QT * r) * H1] =
-        make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
+    col[j + QT * r] = make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
 }
 
 // Inverse rows into G x, then the gradient step:
 //   gv = (1 + c) gx_new - c gx_prev,  u = v + b (gv - aty)   (b = -1/L)
 // with gx <- gx_new.
-__global__ __launch_bounds__(QT) void q1k_c2r_step_kernel(
-    const float2* __restrict__ half, float* __restrict__ gx, const float* __restrict__ v,
+__global__ __launch_bounds__(2 * QT) void q1k_c2r_step_kernel(
+    const float2* __restrict__ halfT, float* __restrict__ gx, const float* __restrict__ v,
     const float* __restrict__ aty, float c1, float c, float b, float* __restrict__ u,
     const float2* __restrict__ tw, const int32_t* pred) {
   if (pred_off(pred)) return;
-  __shared__ float2 lds[Q_LDS];
-  const int j = threadIdx.x;
-  const long long ra = 2ll * blockIdx.x, rb = ra + 1;
+  __shared__ float2 lds[2][Q_LDS];
+  const int g = threadIdx.x >> 8, j = threadIdx.x & (QT - 1);
   QTw t;
-  q_twiddles(t, tw);
-  constexpr int H1 = Q1K / 2 + 1, nh = Q1K / 2;
-  for (int k = j; k < H1; k += QT) {
-    float2 a = half[ra * H1 + k];
-    float2 bb = half[rb * H1 + k];
-    if (k == 0 || k == nh) a.y = bb.y = 0.0f;
-    lds[q_pad(k)] = make_float2(__fsub_rn(a.x, bb.y), __fadd_rn(a.y, bb.x));
-    if (k > 0 && k < nh)
-      lds[q_pad(Q1K - k)] = make_float2(__fadd_rn(a.x, bb.y), __fsub_rn(bb.x, a.y));
-  }
-  __syncthreads();
+  q_twiddles(t, tw, j);
   float2 a[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) a[r] = lds[q_pad(j + QT * r)];
-  q_fft<true>(a, lds, t);
+  q1k_c2r_load(halfT, lds, t, g, j, a);
+  const long long ra = q_row_block() * QR + 2 * g;
   constexpr float sc = 1.0f / Q1K;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const long long e = (h ? rb : ra) * Q1K + j + QT * r;
-      const float gn = __fmul_rn(h ? a[r].y : a[r].x, sc);
+    for (int hh = 0; hh < 2; ++hh) {
+      const long long e = (ra + hh) * Q1K + j + QT * r;
+      const float gn = __fmul_rn(hh ? a[r].y : a[r].x, sc);
       const float gv = __fsub_rn(__fmul_rn(c1, gn), __fmul_rn(c, gx[e]));
       u[e] = __fadd_rn(v[e], __fmul_rn(b, __fsub_rn(gv, aty[e])));
       gx[e] = gn;
@@ -810,10 +815,11 @@ __global__ __launch_bounds__(QT) void q1k_c2r_step_kernel(
   }
 }
 
-// Hermitian part of the (F0, F1) spectrum on the half grid (F0, F1/2 + 1).
+// Hermitian part of the (F0, F1) spectrum on the half grid (F0, F1/2 + 1),
+// stored transposed (hs[k1 F0 + k0]) if trans.
 __global__ __launch_bounds__(256) void hermitian_half_kernel(const float2* __restrict__ h,
                                                              float2* __restrict__ hs, int F0,
-                                                             int F1) {
+                                                             int F1, int trans) {
   const int H1 = F1 / 2 + 1;
   const long long tot = static_cast<long long>(F0) * H1;
   for (long long e = blockIdx.x * 256ll + threadIdx.x; e < tot; e += 256ll * gridDim.x) {
@@ -821,7 +827,8 @@ __global__ __launch_bounds__(256) void hermitian_half_kernel(const float2* __res
     const float2 a = h[static_cast<long long>(k0) * F1 + k1];
     const int m0 = (F0 - k0) % F0, m1 = (F1 - k1) % F1;
     const float2 b = h[static_cast<long long>(m0) * F1 + m1];
-    hs[e] = make_float2(__fmul_rn(0.5f, __fadd_rn(a.x, b.x)), __fmul_rn(0.5f, __fsub_rn(a.y, b.y)));
+    hs[trans ? static_cast<long long>(k1) * F0 + k0 : e] =
+        make_float2(__fmul_rn(0.5f, __fadd_rn(a.x, b.x)), __fmul_rn(0.5f, __fsub_rn(a.y, b.y)));
   }
 }
 
@@ -855,6 +862,9 @@ struct Plan {
   float2* hs = nullptr;
   int lpb_rows = 1, lpb_cols = 1;
 };
+
+// The 1024 x 1024 real path (transposed half spectrum, 256-thread lines).
+bool q1k_path(const Plan& p) { return p.r2c && p.F[0] == Q1K && p.F[1] == Q1K; }
 
 int log2_exact(int n) {
   if (n <= 0 || (n & (n - 1))) return -1;
@@ -963,15 +973,14 @@ int convolve_r2c(const Plan& p, const float* x, float* y, bool adj, const int32_
   const long long R = p.batch * p.F[0];
   float2* half = p.work;  // R rows of H1 (fits in the batch * fsize workspace)
   const long long cl = (R + 1) / 2;  // complex row transforms
-  if (n == Q1K && p.F[0] == Q1K) {
-    hipLaunchKernelGGL(q1k_r2c_kernel, dim3(static_cast<unsigned>(cl)), dim3(QT), 0, s, x, half,
-                       R, t1.tw, pred);
+  if (q1k_path(p)) {
+    const unsigned gr = static_cast<unsigned>(R / QR);
+    hipLaunchKernelGGL(q1k_r2c_kernel, dim3(gr), dim3(2 * QT), 0, s, x, half, t1.tw, pred);
     SCXR_LAUNCHED("fft r2c rows (1024)");
     hipLaunchKernelGGL(q1k_cols_kernel, dim3(static_cast<unsigned>(p.batch * H1)), dim3(QT), 0, s,
-                       half, p.hs, H1, static_cast<int>(adj), p.ax[0].tw, pred);
+                       half, p.hs, static_cast<int>(adj), p.ax[0].tw, pred);
     SCXR_LAUNCHED("fft half-spectrum columns (1024)");
-    hipLaunchKernelGGL(q1k_c2r_kernel, dim3(static_cast<unsigned>(cl)), dim3(QT), 0, s, half, y,
-                       R, t1.tw, pred);
+    hipLaunchKernelGGL(q1k_c2r_kernel, dim3(gr), dim3(2 * QT), 0, s, half, y, t1.tw, pred);
     SCXR_LAUNCHED("fft c2r rows (1024)");
     return SCXR_OK;
   }
@@ -1131,7 +1140,7 @@ int scxr_cconv_set_spectrum(scxr_cconv_plan* plan, const float* h, int32_t h_is_
   }
   if (p.r2c) {
     hipLaunchKernelGGL(hermitian_half_kernel, dim3(256), dim3(256), 0, s, p.h, p.hs, p.F[0],
-                       p.F[1]);
+                       p.F[1], static_cast<int>(q1k_path(p)));
     SCXR_LAUNCHED("cconv half spectrum");
   }
   return SCXR_OK;
@@ -1157,18 +1166,18 @@ int scxr_cconv_apgm_grad(const scxr_cconv_plan* plan, const float* x, const floa
   SCXR_REQUIRE(plan && x && gs && gx && v && aty && u && (!obj || Yh), SCXR_ERR_ARG,
                "null argument to cconv_apgm_grad");
   const Plan& p = plan->p;
-  SCXR_REQUIRE(p.r2c && p.batch == 1 && p.F[0] == Q1K && p.F[1] == Q1K, SCXR_ERR_UNSUPPORTED,
+  SCXR_REQUIRE(q1k_path(p) && p.batch == 1, SCXR_ERR_UNSUPPORTED,
                "fused APGM gradient needs a single 1024 x 1024 real convolution plan");
   const hipStream_t s = as_stream(stream);
   constexpr int H1 = Q1K / 2 + 1;
   float2* half = p.work;
-  hipLaunchKernelGGL(q1k_r2c_kernel, dim3(Q1K / 2), dim3(QT), 0, s, x, half,
-                     static_cast<long long>(Q1K), p.ax[1].tw, pred);
+  hipLaunchKernelGGL(q1k_r2c_kernel, dim3(Q1K / QR), dim3(2 * QT), 0, s, x, half, p.ax[1].tw,
+                     pred);
   SCXR_LAUNCHED("fft r2c rows (1024)");
   hipLaunchKernelGGL(q1k_cols_gram_kernel, dim3(H1), dim3(QT), 0, s, half, p.hs, gs,
-                     reinterpret_cast<const float2*>(Yh), obj, H1, p.ax[0].tw, pred);
+                     reinterpret_cast<const float2*>(Yh), obj, p.ax[0].tw, pred);
   SCXR_LAUNCHED("fft gram columns (1024)");
-  hipLaunchKernelGGL(q1k_c2r_step_kernel, dim3(Q1K / 2), dim3(QT), 0, s, half, gx, v, aty,
+  hipLaunchKernelGGL(q1k_c2r_step_kernel, dim3(Q1K / QR), dim3(2 * QT), 0, s, half, gx, v, aty,
                      1.0f + c, c, b, u, p.ax[1].tw, pred);
   SCXR_LAUNCHED("fft c2r gradient step (1024)");
   return SCXR_OK;

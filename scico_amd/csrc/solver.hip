@@ -588,15 +588,45 @@ __global__ __launch_bounds__(256) void admm_itstat_kernel(
 // One AcceleratedPGM statistics row: hist[row] = {obj_scale * sum(obj) +
 // lam * sum(tv), sqrt(sum(res))} (Objective f + g and Residual,
 // scico/optimize/_pgm.py:151-156, 242-248), row from the device counter.
-__global__ __launch_bounds__(256) void apgm_itstat_kernel(
+// One workgroup of 1024 threads; each thread's share of a partial array is
+// loaded eight at a time before it is added, so the three sums cost about
+// one memory latency each instead of one per element.
+constexpr int ITS_T = 1024;
+__device__ __forceinline__ double its_share(const double* __restrict__ p, int n) {
+  double v = 0.0;
+  for (int e0 = 0; e0 < n; e0 += 8 * ITS_T) {
+    double b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = e0 + static_cast<int>(threadIdx.x) + ITS_T * i;
+      b[i] = e < n ? p[e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v += b[i];
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(ITS_T) void apgm_itstat_kernel(
     const double* __restrict__ obj, int nobj, double obj_scale, const double* __restrict__ tv,
     int ntv, double lam, const double* __restrict__ res, int nres, double* __restrict__ hist,
     int32_t* __restrict__ counter, int hist_cap) {
-  __shared__ double sh[4];
-  const double so = all_sum_f64(obj, nobj, sh);
-  const double st = all_sum_f64(tv, ntv, sh);
-  const double sr = all_sum_f64(res, nres, sh);
+  __shared__ double sh[3][ITS_T / 64];
+  double v[3] = {its_share(obj, nobj), its_share(tv, ntv), its_share(res, nres)};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    if (lane == 0) sh[q][wave] = v[q];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double so = 0.0, st = 0.0, sr = 0.0;
+    for (int w = 0; w < ITS_T / 64; ++w) {
+      so += sh[0][w];
+      st += sh[1][w];
+      sr += sh[2][w];
+    }
     const int row = counter[0];
     if (row < hist_cap) {
       hist[2 * row] = obj_scale * so + lam * st;
@@ -716,40 +746,47 @@ __global__ __launch_bounds__(256) void tv_prox_kernel(const float* __restrict__ 
 // coefficients at p, p - e0 and p - e1 with the same operations, in the
 // same order, as tv_prox_kernel (so the results are bitwise equal).
 //
-// APGM: the AcceleratedPGM step that follows the prox (scico/optimize/
-// _pgm.py:221-240) fused into the store: with out holding x_prev and vm the
-// previous extrapolated point v, each output x also gives the partial sum
-// of (x - v)^2 (the fixed-point residual, float64, one per workgroup) and
-// v <- x + c (x - x_prev), with the operations of the unfused path.
+// APGM (EXT = 1): the AcceleratedPGM step that follows the prox
+// (scico/optimize/_pgm.py:221-240) and the TV value of its result, fused.
+// The tile computes one extra row and column of outputs (the first of the
+// next tiles, bitwise as they compute them) so that every pixel's forward
+// differences are at hand.  With out holding x_prev and vm the previous
+// extrapolated point v, each stored output x gives the partial sums of
+// (x - v)^2 (the fixed-point residual) and of the TV norm of x (as
+// tv_norm_kernel: sqrt of the squared differences or their absolute
+// values, float64), one each per workgroup, and v <- x + c (x - x_prev),
+// with the operations of the unfused path.
 constexpr int TVT = 16;
-template <bool APGM>
+template <int EXT>
 __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict__ v,
                                                         float* __restrict__ out, TvDims t,
                                                         float tau, float invK, int iso,
                                                         const float* __restrict__ w,
                                                         const int32_t* pred,
                                                         float* __restrict__ vm, float c,
-                                                        double* __restrict__ res_ws) {
+                                                        double* __restrict__ res_ws,
+                                                        double* __restrict__ tv_ws) {
   if (pred_off(pred)) return;
-  __shared__ float vs[TVT + 2][TVT + 3];
-  __shared__ float cH[2][TVT + 1][TVT + 2], cL[2][TVT + 1][TVT + 2];
+  constexpr int NO = TVT + EXT;  // outputs per tile side
+  __shared__ float vs[NO + 2][NO + 3];
+  __shared__ float cH[2][NO + 1][NO + 2], cL[2][NO + 1][NO + 2];
   const float s = 0.70710677f;
   const int M = t.d[1], N = t.d[2], P0 = t.P[1], P1 = t.P[2];
   const int i0 = blockIdx.y * TVT, j0 = blockIdx.x * TVT;
   const int tid = threadIdx.x;
   auto wrap = [](int q, int P) { q %= P; return q < 0 ? q + P : q; };
-  for (int e = tid; e < (TVT + 2) * (TVT + 2); e += 256) {
-    const int r = e / (TVT + 2), c = e - r * (TVT + 2);
-    const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + c, P1);
-    vs[r][c] = v[static_cast<int64_t>(min(qi, M - 1)) * N + min(qj, N - 1)];
+  for (int e = tid; e < (NO + 2) * (NO + 2); e += 256) {
+    const int r = e / (NO + 2), cc = e - r * (NO + 2);
+    const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + cc, P1);
+    vs[r][cc] = v[static_cast<int64_t>(min(qi, M - 1)) * N + min(qj, N - 1)];
   }
   __syncthreads();
-  for (int e = tid; e < (TVT + 1) * (TVT + 1); e += 256) {
-    const int r = e / (TVT + 1), c = e - r * (TVT + 1);
-    const float vq = vs[r][c], vn0 = vs[r + 1][c], vn1 = vs[r][c + 1];
+  for (int e = tid; e < (NO + 1) * (NO + 1); e += 256) {
+    const int r = e / (NO + 1), cc = e - r * (NO + 1);
+    const float vq = vs[r][cc], vn0 = vs[r + 1][cc], vn1 = vs[r][cc + 1];
     float H[2] = {__fmul_rn(s, __fsub_rn(vn0, vq)), __fmul_rn(s, __fsub_rn(vn1, vq))};
     const float L0 = __fmul_rn(s, __fadd_rn(vq, vn0)), L1 = __fmul_rn(s, __fadd_rn(vq, vn1));
-    const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + c, P1);
+    const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + cc, P1);
     if (t.circ || (qi < M && qj < N)) {
       const float tq = w ? __fmul_rn(tau, w[static_cast<int64_t>(qi) * N + qj]) : tau;
       if (iso) {
@@ -759,36 +796,58 @@ __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict_
         H[1] = l1_shrink(H[1], tq);
       }
     }
-    cH[0][r][c] = H[0];
-    cH[1][r][c] = H[1];
-    cL[0][r][c] = L0;
-    cL[1][r][c] = L1;
+    cH[0][r][cc] = H[0];
+    cH[1][r][cc] = H[1];
+    cL[0][r][cc] = L0;
+    cL[1][r][cc] = L1;
   }
   __syncthreads();
-  const int ty = tid / TVT, tx = tid - ty * TVT;
-  const int pi = i0 + ty, pj = j0 + tx;
-  double res = 0.0;
-  if (pi < M && pj < N) {
-    // p at (ty + 1, tx + 1); p - e0 at (ty, tx + 1); p - e1 at (ty + 1, tx)
+  // output (ty, tx) of the tile: p at (ty + 1, tx + 1), p - e0 at
+  // (ty, tx + 1), p - e1 at (ty + 1, tx)
+  auto output = [&](int ty, int tx) {
     const float lo0 = __fmul_rn(s, __fadd_rn(cL[0][ty + 1][tx + 1], cL[0][ty][tx + 1]));
     const float hi0 = __fmul_rn(s, __fsub_rn(cH[0][ty][tx + 1], cH[0][ty + 1][tx + 1]));
     const float lo1 = __fmul_rn(s, __fadd_rn(cL[1][ty + 1][tx + 1], cL[1][ty + 1][tx]));
     const float hi1 = __fmul_rn(s, __fsub_rn(cH[1][ty + 1][tx], cH[1][ty + 1][tx + 1]));
     const float acc = __fadd_rn(__fadd_rn(lo0, hi0), __fadd_rn(lo1, hi1));
-    const float xn = __fmul_rn(invK, acc);
-    const int64_t e = static_cast<int64_t>(pi) * N + pj;
-    if constexpr (APGM) {
+    return __fmul_rn(invK, acc);
+  };
+  const int ty = tid / TVT, tx = tid - ty * TVT;
+  const int pi = i0 + ty, pj = j0 + tx;
+  if constexpr (EXT == 0) {
+    if (pi < M && pj < N) out[static_cast<int64_t>(pi) * N + pj] = output(ty, tx);
+  } else {
+    __shared__ float xs[NO][NO + 1];
+    for (int e = tid; e < NO * NO; e += 256) {
+      const int r = e / NO, cc = e - r * NO;
+      xs[r][cc] = output(r, cc);
+    }
+    __syncthreads();
+    double res = 0.0, tv = 0.0;
+    if (pi < M && pj < N) {
+      const float xn = xs[ty][tx];
+      const int64_t e = static_cast<int64_t>(pi) * N + pj;
       const float xo = out[e], vo = vm[e];
       const double d = static_cast<double>(xn) - static_cast<double>(vo);
       res = d * d;
       vm[e] = __fadd_rn(xn, __fmul_rn(c, __fsub_rn(xn, xo)));
+      out[e] = xn;
+      const bool circ = t.circ != 0;
+      const float g0 = fd::diff_fwd(xn, xs[ty + 1][tx], pi, M, circ);
+      const float g1 = fd::diff_fwd(xn, xs[ty][tx + 1], pj, N, circ);
+      if (iso)
+        tv = sqrt(static_cast<double>(g0) * g0 + static_cast<double>(g1) * g1);
+      else
+        tv = fabs(static_cast<double>(g0)) + fabs(static_cast<double>(g1));
     }
-    out[e] = xn;
-  }
-  if constexpr (APGM) {
     __shared__ double sh[4];
     res = block_sum_f64(res, sh);
-    if (tid == 0) res_ws[blockIdx.y * gridDim.x + blockIdx.x] = res;
+    __syncthreads();
+    tv = block_sum_f64(tv, sh);
+    if (tid == 0) {
+      res_ws[blockIdx.y * gridDim.x + blockIdx.x] = res;
+      tv_ws[blockIdx.y * gridDim.x + blockIdx.x] = tv;
+    }
   }
 }
 
@@ -1283,7 +1342,8 @@ int scxr_apgm_itstat(const double* obj, int32_t nobj, double obj_scale, const do
   SCXR_REQUIRE(obj && tv && res && hist && counter && nobj > 0 && ntv > 0 && nres > 0 &&
                    hist_cap > 0,
                SCXR_ERR_ARG, "bad apgm_itstat arguments");
-  hipLaunchKernelGGL(solver::apgm_itstat_kernel, dim3(1), dim3(256), 0, as_stream(stream), obj,
+  hipLaunchKernelGGL(solver::apgm_itstat_kernel, dim3(1), dim3(solver::ITS_T), 0,
+                     as_stream(stream), obj,
                      nobj, obj_scale, tv, ntv, lam, res, nres, hist, counter, hist_cap);
   SCXR_LAUNCHED("apgm itstat");
   return SCXR_OK;
@@ -1311,8 +1371,9 @@ int scxr_tv_prox(int32_t ndim, const int32_t* shape, const float* v, float* out,
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
   if (ndim == 2) {
     const dim3 g(ceil_div(t.d[2], solver::TVT), ceil_div(t.d[1], solver::TVT));
-    hipLaunchKernelGGL(solver::tv_prox2d_kernel<false>, g, dim3(256), 0, as_stream(stream), v,
-                       out, t, tau, 0.25f, isotropic, nullptr, pred, nullptr, 0.0f, nullptr);
+    hipLaunchKernelGGL(solver::tv_prox2d_kernel<0>, g, dim3(256), 0, as_stream(stream), v,
+                       out, t, tau, 0.25f, isotropic, nullptr, pred, nullptr, 0.0f, nullptr,
+                       nullptr);
   } else {
     hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                        as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
@@ -1329,16 +1390,16 @@ int32_t scxr_tv_prox_apgm_parts(int32_t M, int32_t N) {
 
 int scxr_tv_prox_apgm(int32_t M, int32_t N, const float* u, float* x, float* v, float tau,
                       int32_t isotropic, int32_t circular, float c, double* res_ws,
-                      const int32_t* pred, void* stream) {
-  SCXR_REQUIRE(u && x && v && res_ws && u != x && u != v && x != v, SCXR_ERR_ARG,
+                      double* tv_ws, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(u && x && v && res_ws && tv_ws && u != x && u != v && x != v, SCXR_ERR_ARG,
                "bad tv_prox_apgm arguments (u, x and v must be distinct)");
   const int32_t shape[2] = {M, N};
   solver::TvDims t;
   const int rc = tv_dims(2, shape, circular, t);
   if (rc) return rc;
   const dim3 g(ceil_div(N, solver::TVT), ceil_div(M, solver::TVT));
-  hipLaunchKernelGGL(solver::tv_prox2d_kernel<true>, g, dim3(256), 0, as_stream(stream), u, x, t,
-                     tau, 0.25f, isotropic, nullptr, pred, v, c, res_ws);
+  hipLaunchKernelGGL(solver::tv_prox2d_kernel<1>, g, dim3(256), 0, as_stream(stream), u, x, t,
+                     tau, 0.25f, isotropic, nullptr, pred, v, c, res_ws, tv_ws);
   SCXR_LAUNCHED("tv prox apgm");
   return SCXR_OK;
 }
@@ -1354,8 +1415,8 @@ int scxr_tv_prox_w(int32_t ndim, const int32_t* shape, const float* v, const flo
   const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
   if (ndim == 2) {
     const dim3 g(ceil_div(t.d[2], solver::TVT), ceil_div(t.d[1], solver::TVT));
-    hipLaunchKernelGGL(solver::tv_prox2d_kernel<false>, g, dim3(256), 0, as_stream(stream), v,
-                       out, t, tau, 0.25f, isotropic, w, pred, nullptr, 0.0f, nullptr);
+    hipLaunchKernelGGL(solver::tv_prox2d_kernel<0>, g, dim3(256), 0, as_stream(stream), v,
+                       out, t, tau, 0.25f, isotropic, w, pred, nullptr, 0.0f, nullptr, nullptr);
   } else {
     hipLaunchKernelGGL(solver::tv_prox_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                        as_stream(stream), v, out, t, tau, 1.0f / static_cast<float>(2 * ndim),
@@ -1390,19 +1451,6 @@ int scxr_tv_prox_vjp(int32_t ndim, const int32_t* shape, const float* v, const f
   hipLaunchKernelGGL(solver::tv_vjp_v_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
                      as_stream(stream), HB, LB, t, gv);
   SCXR_LAUNCHED("tv prox vjp v");
-  return SCXR_OK;
-}
-
-int scxr_tv_norm_partials(int32_t ndim, const int32_t* shape, const float* x, int32_t isotropic,
-                          int32_t circular, double* ws, const int32_t* pred, void* stream) {
-  SCXR_REQUIRE(x && ws, SCXR_ERR_ARG, "bad tv_norm_partials arguments");
-  solver::TvDims t;
-  const int rc = tv_dims(ndim, shape, circular, t);
-  if (rc) return rc;
-  const int64_t n = static_cast<int64_t>(t.d[0]) * t.d[1] * t.d[2];
-  hipLaunchKernelGGL(solver::tv_norm_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
-                     as_stream(stream), x, t, isotropic, ws, pred);
-  SCXR_LAUNCHED("tv norm partials");
   return SCXR_OK;
 }
 

@@ -11,8 +11,8 @@ structure
     constant step size,
 
 the iteration runs through fused kernels (:class:`_FusedDeconv`): the
-prox writes x, the residual partials and the extrapolated point in one
-pass, and one FFT convolution by ``G = A^T A`` per iteration gives both
+prox writes x, the partials of the residual and of TV(x) and the
+extrapolated point in one pass, and one FFT convolution by ``G = A^T A`` per iteration gives both
 the next gradient (``G v = (1 + c) G x - c G x_prev`` by linearity) and,
 on the way through the spectrum, the objective (Parseval).  The reference
 evaluates three convolutions per iteration (``A v``, ``A^T r`` and
@@ -21,7 +21,6 @@ evaluates three convolutions per iteration (``A v``, ``A^T r`` and
 
 from __future__ import annotations
 
-import ctypes
 from typing import List
 
 import numpy as np
@@ -199,21 +198,22 @@ class _FusedDeconv:
         hneg = np.roll(np.flip(hd, (0, 1)), 1, (0, 1))
         hs = 0.5 * (hd + np.conj(hneg))
         H1 = self.N // 2 + 1
-        self.gs = dev.as_device(np.ascontiguousarray((hs.real**2 + hs.imag**2)[:, :H1],
+        # half-grid arrays transposed (column k1 contiguous), as the kernels
+        # keep the half spectrum
+        self.gs = dev.as_device(np.ascontiguousarray((hs.real**2 + hs.imag**2)[:, :H1].T,
                                                      dtype=np.float32))
         yh = np.fft.rfft2(dev.to_numpy(y).astype(np.float64))
-        self.Yh = torch.as_tensor(np.ascontiguousarray(yh.astype(np.complex64)),
+        self.Yh = torch.as_tensor(np.ascontiguousarray(yh.T.astype(np.complex64)),
                                   device=dev.DEVICE)
         self.aty = self.A.adj(y)
         self.n_obj = H1
         self.n_res = int(lib.scxr_tv_prox_apgm_parts(self.M, self.N))
-        self.n_tv = int(lib.scxr_elem_blocks(n))
+        self.n_tv = self.n_res  # the prox kernel's TV partials, one per workgroup
         self.ws_obj = torch.empty(self.n_obj, **f64)
         self.ws_res = torch.empty(self.n_res, **f64)
         self.ws_tv = torch.empty(self.n_tv, **f64)
         self.u = torch.empty((self.M, self.N), **f32)
         self.gx = torch.zeros((self.M, self.N), **f32)
-        self.shape_c = (ctypes.c_int32 * 2)(self.M, self.N)
         self.obj_scale = 0.5 / float(n)
         self.hist_cap = 0
         self.hist = None
@@ -282,9 +282,7 @@ class _FusedDeconv:
         P = lambda t: t.data_ptr()
         s = dev.stream()
         call("scxr_tv_prox_apgm", self.M, self.N, P(self.u), P(self.x), P(self.v), tau, self.iso,
-             self.circ, c, P(self.ws_res), None, s)
-        call("scxr_tv_norm_partials", 2, self.shape_c, P(self.x), self.iso, self.circ,
-             P(self.ws_tv), None, s)
+             self.circ, c, P(self.ws_res), P(self.ws_tv), None, s)
         call("scxr_cconv_apgm_grad", self.A.plan, P(self.x), P(self.gs), P(self.Yh),
              P(self.ws_obj), P(self.gx), P(self.v), P(self.aty), c, b, P(self.u), None, s)
         call("scxr_apgm_itstat", P(self.ws_obj), self.n_obj, self.obj_scale, P(self.ws_tv),
