@@ -90,7 +90,7 @@ def _running_max(v):
 
 @pytest.mark.parametrize("name,regime,fused", [
     ("c1", "configured", True), ("c1", "configured", False), ("c1", "fixedwork", True),
-    ("c2", "configured", True)])
+    ("c2", "configured", True), ("c2", "fixedwork", True)])
 def test_admm_free_running_vs_fixture(gpu, name, regime, fused):
     fx = _fixture(name, regime)
     cfg, ang, xgt, g, y = _problem(name)
