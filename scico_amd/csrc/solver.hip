@@ -772,9 +772,28 @@ __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict_
   __shared__ float cH[2][NO + 1][NO + 2], cL[2][NO + 1][NO + 2];
   const float s = 0.70710677f;
   const int M = t.d[1], N = t.d[2], P0 = t.P[1], P1 = t.P[2];
-  const int i0 = blockIdx.y * TVT, j0 = blockIdx.x * TVT;
+  // tiles in row-major order, consecutive ones on one XCD (workgroups go to
+  // the 8 XCDs in turn): neighbouring tiles share the cache lines of their
+  // rows and halos (a bijection of the first 8 * floor(G / 8) workgroups)
+  int bx = blockIdx.x, by = blockIdx.y;
+  {
+    const int G = gridDim.x * gridDim.y, G8 = (G / 8) * 8;
+    int L = by * gridDim.x + bx;
+    if (L < G8) L = (L % 8) * (G8 / 8) + L / 8;
+    by = L / gridDim.x;
+    bx = L - by * gridDim.x;
+  }
+  const int i0 = by * TVT, j0 = bx * TVT;
   const int tid = threadIdx.x;
-  auto wrap = [](int q, int P) { q %= P; return q < 0 ? q + P : q; };
+  // q >= -1 here; the modulo is left for images smaller than a tile
+  auto wrap = [](int q, int P) {
+    if (q < 0) return q + P;
+    if (q >= P) {
+      q -= P;
+      if (q >= P) q %= P;
+    }
+    return q;
+  };
   for (int e = tid; e < (NO + 2) * (NO + 2); e += 256) {
     const int r = e / (NO + 2), cc = e - r * (NO + 2);
     const int qi = wrap(i0 - 1 + r, P0), qj = wrap(j0 - 1 + cc, P1);
@@ -845,8 +864,8 @@ __global__ __launch_bounds__(256) void tv_prox2d_kernel(const float* __restrict_
     __syncthreads();
     tv = block_sum_f64(tv, sh);
     if (tid == 0) {
-      res_ws[blockIdx.y * gridDim.x + blockIdx.x] = res;
-      tv_ws[blockIdx.y * gridDim.x + blockIdx.x] = tv;
+      res_ws[by * gridDim.x + bx] = res;
+      tv_ws[by * gridDim.x + bx] = tv;
     }
   }
 }
