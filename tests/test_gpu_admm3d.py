@@ -230,3 +230,62 @@ def test_graph_replay_2d(gpu):
                                                                         "maxiter": 10}))
 
     _graph_replay_matches_eager(make)
+
+
+def test_admm_tv3d_slab_config_length(gpu):
+    """The 3D configuration at full iteration count on its central 16-slice
+    slab (256 x 16 x 256 volume, 360 views about axis 1, detector 368 x 16,
+    anisotropic TV, CG 10 steps with tol 1e-4, 50 iterations) against the
+    oracle's run (``tests/golden/admm3d_slab.npz``, written by
+    ``tests/golden/make_admm3d_fixture.py``): every statistics column within
+    twice the per-iteration floor envelope (running maximum; residual
+    columns scaled by their running maximum; plus 1e-6 of the statistic,
+    float32 resolution), the pooled final volume and the SNR within twice
+    their floors, CG counts differing on at most twice as many iterations
+    as the other valid runs'."""
+    import json
+    import os
+    import sys
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    if gold not in sys.path:
+        sys.path.insert(0, gold)
+    import make_admm3d_fixture as F
+
+    d = np.load(os.path.join(gold, "admm3d_slab.npz"), allow_pickle=False)
+    fx = {k: d[k] for k in d.files}
+    meta = json.loads(str(fx["meta"]))
+    M, xgt, y = F.problem()
+    assert F.sha256(y) == meta["y_sha256"], "regenerated projections differ from the fixture's"
+    shape = (F.N, F.SLAB, F.N)
+    H = XRayTransform3D(shape, M, F.DET)
+    assert H.info()["mode"] == "slice"
+    s = ADMM(f=loss.SquaredL2Loss(y=dev(y), A=H), g_list=[F.LAM * functional.L1Norm()],
+             C_list=[linop.FiniteDifference(shape, append=0)], rho_list=[F.RHO],
+             maxiter=F.ITERS, subproblem_solver=LinearSubproblemSolver(
+                 cg_kwargs={"tol": F.CG_TOL, "maxiter": F.CG_MAXITER}))
+    assert s._fused is not None
+    x = s.solve().cpu().numpy()
+    h = s.history(transpose=True)
+    hr = fx["hist"]
+    sc = F.scales(hr)
+    worst = {}
+    for col, key, field in ((1, "objective", "Objective"), (2, "prml", "Prml_Rsdl"),
+                            (3, "dual", "Dual_Rsdl"), (5, "cg_res", "CG_Res")):
+        got = np.asarray(getattr(h, field), dtype=np.float64)
+        dev_ = np.abs(got - hr[:, col]) / sc[key]
+        bar = 2 * np.maximum.accumulate(fx[f"floor_{key}"]) + 1e-6 * np.abs(hr[:, col]) / sc[key]
+        worst[key] = float(np.max(dev_ / bar))
+        over = np.nonzero(dev_ > bar)[0]
+        assert len(over) == 0, (key, [(int(k), dev_[k], bar[k]) for k in over[:8]])
+    cg_diff = int(np.sum(np.asarray(h.CG_It, np.int64) != hr[:, 4].astype(np.int64)))
+    fin = meta["floor_final"]
+    snr = F.snr(x, xgt)
+    e_pool = rel_l2(F.pool(x), fx["x_pool"])
+    print(f"  3D slab x{F.ITERS}: worst deviation / bar {worst}; CG counts differ on {cg_diff} "
+          f"(variants up to {fin['cg_count_diffs']}); SNR {snr:.3f} vs {meta['snr_db']:.3f} dB "
+          f"(floor {fin['snr_abs']:.3f}); pooled volume rel-L2 {e_pool:.2e} (floor "
+          f"{fin['x_pool_rel_l2']:.2e})", flush=True)
+    assert cg_diff <= 2 * fin["cg_count_diffs"]
+    assert abs(snr - meta["snr_db"]) <= 2 * fin["snr_abs"] + 1e-3
+    assert e_pool <= 2 * fin["x_pool_rel_l2"] + 1e-7
