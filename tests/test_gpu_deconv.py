@@ -275,7 +275,7 @@ def test_apgm_deconvolution_config_length(gpu, fused):
     amplify rounding the way ten-step CG on the CT normal equations does:
     the other valid float32 runs (float64 FFTs, FFT-rounding-sized noise)
     end 1.4e-7 (rel-L2) from the oracle, so the reconstruction is held to
-    the north star's 1e-5 directly; the objective and residual statistics
+    the parity bar of 1e-5 (rel-L2) directly; the objective and residual statistics
     to 1e-5 of their value (residual: of its largest value so far).  Both
     the fused iteration (one convolution per iteration: the gradient by
     linearity, the objective by Parseval) and the generic one."""
