@@ -161,9 +161,28 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
   constexpr int RPP = 64 / CPR;
   constexpr int BIG = INT_MAX / 2;
   const int lane = threadIdx.x & 63;
-  const int v = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // Workgroup order: consecutive workgroups go to the 8 XCDs in turn; each
+  // XCD is given a contiguous run of an order that takes the view groups
+  // in blocks of about an eighth of them, every bin group of a block with
+  // its views adjacent.  Each XCD then works on its own range of views
+  // (neighbouring views' strips share most of their pixel rows, so they
+  // share its L2) across all bin groups (whose strips differ in length,
+  // so every XCD gets a similar share of work).  A bijection of the grid.
+  int bg, vg;
+  {
+    const int GX = gridDim.x, GY = gridDim.y, G = GX * GY, G8 = (G / 8) * 8;
+    int L = blockIdx.y * GX + blockIdx.x;
+    if (L < G8) L = (L % 8) * (G8 / 8) + L / 8;
+    const int VB = GY >= 8 ? GY / 8 : 1;  // view groups per block
+    const int blk = L / (GX * VB);
+    const int nb = min(VB, GY - blk * VB);  // the last block may be short
+    const int within = L - blk * GX * VB;
+    bg = within / nb;
+    vg = blk * VB + (within - bg * nb);
+  }
+  const int v = vg * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (v >= V) return;
-  const int u0 = blockIdx.x * FB;
+  const int u0 = bg * FB;
   const unsigned quad = blockIdx.z * 64 + lane;
   const bool qok = quad < static_cast<unsigned>(SQ);
   const unsigned qoff = (qok ? quad : 0u) * 16u;
