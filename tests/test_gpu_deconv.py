@@ -389,3 +389,42 @@ def test_fused_apgm_first_step_and_state(gpu):
     print(f"  fused APGM 1024^2: 30 steps vs generic rel-L2 {e30:.2e}; restored run "
           f"{e_restore:.2e}; graph replay bitwise {same}", flush=True)
     assert e30 <= 1e-6 and e_restore <= 1e-6 and same
+
+
+@pytest.mark.parametrize("iso,circular", [(True, True), (False, True), (True, False)])
+def test_fused_apgm_tv_variants(gpu, iso, circular):
+    """The fused deconvolution iteration with isotropic / anisotropic and
+    circular / non-circular TV against the generic iteration: the first
+    step bit for bit, 10 steps to 1e-6, statistics to 1e-6."""
+    from scico_amd.optimize import _pgm
+
+    xgt, h, hd, y, L0 = _deconv_problem(1024)
+    A = CircularConvolve(h, (1024, 1024), h_center=[7, 7])
+    f = loss.SquaredL2Loss(y=dev(y), A=A)
+    tv = functional.IsotropicTVNorm(circular=circular) if iso else \
+        functional.AnisotropicTVNorm(circular=circular)
+    g = 2e-2 * tv
+    sols = {}
+    for fused in (True, False):
+        old = _pgm._FusedDeconv.enabled
+        _pgm._FusedDeconv.enabled = fused
+        try:
+            s1 = AcceleratedPGM(f=f, g=g, L0=L0, x0=dev(np.zeros((1024, 1024), np.float32)),
+                                maxiter=1)
+            s10 = AcceleratedPGM(f=f, g=g, L0=L0, x0=dev(np.zeros((1024, 1024), np.float32)),
+                                 maxiter=10)
+        finally:
+            _pgm._FusedDeconv.enabled = old
+        assert (s1._fused is not None) == fused
+        s1.solve()
+        x10 = s10.solve().cpu().numpy()
+        sols[fused] = (s1.x.clone(), x10, s10.history(transpose=True))
+    assert torch.equal(sols[True][0], sols[False][0])
+    e = rel_l2(sols[True][1], sols[False][1])
+    hf, hg = sols[True][2], sols[False][2]
+    d_obj = max(abs(float(a) - float(b)) / abs(float(b)) for a, b in zip(hf.Objective, hg.Objective))
+    d_res = max(abs(float(a) - float(b)) / float(hg.Residual[0])
+                for a, b in zip(hf.Residual, hg.Residual))
+    print(f"  fused APGM iso={iso} circ={circular}: 10 steps rel-L2 {e:.2e}, objective "
+          f"{d_obj:.2e}, residual {d_res:.2e}", flush=True)
+    assert e <= 1e-6 and d_obj <= 1e-6 and d_res <= 1e-6
