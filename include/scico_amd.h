@@ -290,14 +290,15 @@ int scxr_tv_norm(int32_t ndim, const int32_t* shape, const float* x, int32_t iso
 /* ------------------------------- fused AcceleratedPGM deconvolution --
  * One AcceleratedPGM-TV iteration (scico/optimize/_pgm.py:183-252) on a
  * 1024 x 1024 image with f = 0.5 ||A x - y||^2, A a real CircularConvolve,
- * g = lam TV (2D), constant step 1/L, in five launches and one statistics
- * row:
+ * g = lam TV (2D), constant step 1/L, in four launches that also record
+ * its statistics row:
  *   scxr_tv_prox_apgm:  x <- prox_{tau TV}(u) (tau = lam / L), the
  *     partials of ||x - v||^2 and of TV(x), v <- x + c (x - x_prev)
  *     (c = (t_k - 1) / t_{k+1});
- *   scxr_cconv_apgm_grad: G x and the next gradient step u (see there),
- *     with the objective's partials;
- *   scxr_apgm_itstat: the Objective and Residual row.
+ *   scxr_cconv_apgm_grad_itstat: G x and the next gradient step u, with
+ *     the objective's partials, and the Objective and Residual row (the
+ *     three launches of scxr_cconv_apgm_grad, the last of which also
+ *     reduces the row as scxr_apgm_itstat does).
  */
 /* Workgroup (partial) count of scxr_tv_prox_apgm for an M x N image. */
 int32_t scxr_tv_prox_apgm_parts(int32_t M, int32_t N);
@@ -359,6 +360,20 @@ int scxr_cconv_apgm_grad(const scxr_cconv_plan* plan, const float* x, const floa
                          const float* Yh, double* obj, float* gx, const float* v,
                          const float* aty, float c, float b, float* u, const int32_t* pred,
                          void* stream);
+/* scxr_cconv_apgm_grad with obj and Yh required, followed in its last
+ * launch by the statistics row of scxr_apgm_itstat(obj, 513, obj_scale,
+ * tv, ntv, lam, res, nres, hist, counter, hist_cap): the row sums the
+ * objective partials of this call and the prox partials (tv, res) that
+ * earlier work on the stream wrote.  The sums' double roundings may differ
+ * from scxr_apgm_itstat's (512 threads instead of 1024).  A call whose
+ * predicate is 0 records no row.  No reference counterpart as one call
+ * (see scxr_cconv_apgm_grad; the row is _pgm.py:242-248's itstat). */
+int scxr_cconv_apgm_grad_itstat(const scxr_cconv_plan* plan, const float* x, const float* gs,
+                                const float* Yh, double* obj, float* gx, const float* v,
+                                const float* aty, float c, float b, float* u, double obj_scale,
+                                const double* tv, int32_t ntv, double lam, const double* res,
+                                int32_t nres, double* hist, int32_t* counter, int32_t hist_cap,
+                                const int32_t* pred, void* stream);
 /* Complex fftn / ifftn over the plan's F axes (x real if x_real != 0). */
 int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,
               int32_t inverse, const int32_t* pred, void* stream);

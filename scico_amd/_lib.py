@@ -156,6 +156,10 @@ SIGNATURES = [
     ("scxr_cconv_apgm_grad", c_int32,
      [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
       c_float, c_void_p, c_void_p, c_void_p]),
+    ("scxr_cconv_apgm_grad_itstat", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+      c_float, c_void_p, c_double, c_void_p, c_int32, c_double, c_void_p, c_int32, c_void_p,
+      c_void_p, c_int32, c_void_p, c_void_p]),
     # CircularConvolve / FFT
     ("scxr_cconv_plan_create", c_int32, [c_int32, c_int32_p, c_int64, POINTER(c_void_p)]),
     ("scxr_cconv_plan_destroy", c_int32, [c_void_p]),

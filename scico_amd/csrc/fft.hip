@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "apgm_stats.h"
 #include "common.h"
 
 using namespace scxr;
@@ -600,13 +601,14 @@ __device__ __forceinline__ void q_fft(float2 (&a)[4], float2* lds, const QTw& t,
 constexpr int QR = 4;
 constexpr int QH1 = Q1K / 2 + 1;
 
-// Row block of a row-kernel workgroup: consecutive workgroups go to the 8
+// Row block of a row-kernel workgroup whose grid holds `first` other
+// workgroups before its G row blocks: consecutive workgroups go to the 8
 // XCDs in turn, so each XCD takes a run of neighbouring row blocks, whose
 // pieces of one transposed column share cache lines (a bijection of the
-// first 8 * floor(G / 8) workgroups; the rest keep their own block).
-__device__ __forceinline__ long long q_row_block() {
-  long long b = blockIdx.x;
-  const long long G8 = (static_cast<long long>(gridDim.x) / 8) * 8;
+// first 8 * floor(G / 8) row workgroups; the rest keep their own block).
+__device__ __forceinline__ long long q_row_block(int first = 0) {
+  long long b = static_cast<long long>(blockIdx.x) - first;
+  const long long G8 = ((static_cast<long long>(gridDim.x) - first) / 8) * 8;
   if (b < G8) b = (b % 8) * (G8 / 8) + b / 8;
   return b;
 }
@@ -680,9 +682,12 @@ __global__ __launch_bounds__(QT) void q1k_cols_kernel(float2* __restrict__ halfT
 // 1/1024 (Z[k] = X_a[k] + i X_b[k] and its Hermitian extension, imaginary
 // parts of k = 0 and 512 dropped as irfft does).  Loads the transform
 // result of thread j of group g into a (pair of rows 2g, 2g + 1).
+// Rows r0 .. r0 + QR - 1 (r0 = QR rb) of the half spectrum, inverse
+// transformed: a holds the two real rows r0 + 2 g, r0 + 2 g + 1.
 __device__ __forceinline__ void q1k_c2r_load(const float2* __restrict__ halfT, float2 (&lds)[2][Q_LDS],
-                                             const QTw& t, int g, int j, float2 (&a)[4]) {
-  const long long r0 = q_row_block() * QR;
+                                             long long rb, const QTw& t, int g, int j,
+                                             float2 (&a)[4]) {
+  const long long r0 = rb * QR;
   const long long item = r0 / Q1K;
   const int rl = static_cast<int>(r0 - item * Q1K);
   const float4* in = reinterpret_cast<const float4*>(halfT + item * QH1 * Q1K + rl);
@@ -712,8 +717,9 @@ __global__ __launch_bounds__(2 * QT) void q1k_c2r_kernel(const float2* __restric
   QTw t;
   q_twiddles(t, tw, j);
   float2 a[4];
-  q1k_c2r_load(halfT, lds, t, g, j, a);
-  const long long ra = q_row_block() * QR + 2 * g;
+  const long long rb = q_row_block();
+  q1k_c2r_load(halfT, lds, rb, t, g, j, a);
+  const long long ra = rb * QR + 2 * g;
   constexpr float sc = 1.0f / Q1K;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -786,21 +792,46 @@ __global__ __launch_bounds__(QT) void q1k_cols_gram_kernel(float2* __restrict__ 
     col[j + QT * r] = make_float2(__fmul_rn(a[r].x, sc), __fmul_rn(a[r].y, sc));
 }
 
+// Statistics of an AcceleratedPGM iteration (apgm_stats_block arguments).
+struct ApgmStats {
+  const double* obj;
+  int nobj;
+  double obj_scale;
+  const double* tv;
+  int ntv;
+  double lam;
+  const double* res;
+  int nres;
+  double* hist;
+  int32_t* counter;
+  int hist_cap;
+};
+
 // Inverse rows into G x, then the gradient step:
 //   gv = (1 + c) gx_new - c gx_prev,  u = v + b (gv - aty)   (b = -1/L)
-// with gx <- gx_new.
+// with gx <- gx_new.  With STATS, workgroup 0 reduces the iteration's
+// statistics row instead (from partials that the kernels before this one
+// on the stream wrote) while workgroups 1 .. Q1K / QR take the row blocks,
+// so the row needs no launch of its own.
+template <bool STATS>
 __global__ __launch_bounds__(2 * QT) void q1k_c2r_step_kernel(
     const float2* __restrict__ halfT, float* __restrict__ gx, const float* __restrict__ v,
     const float* __restrict__ aty, float c1, float c, float b, float* __restrict__ u,
-    const float2* __restrict__ tw, const int32_t* pred) {
+    const float2* __restrict__ tw, ApgmStats st, const int32_t* pred) {
   if (pred_off(pred)) return;
+  if (STATS && blockIdx.x == 0) {
+    apgm_stats_block<2 * QT>(st.obj, st.nobj, st.obj_scale, st.tv, st.ntv, st.lam, st.res,
+                             st.nres, st.hist, st.counter, st.hist_cap);
+    return;
+  }
   __shared__ float2 lds[2][Q_LDS];
   const int g = threadIdx.x >> 8, j = threadIdx.x & (QT - 1);
   QTw t;
   q_twiddles(t, tw, j);
   float2 a[4];
-  q1k_c2r_load(halfT, lds, t, g, j, a);
-  const long long ra = q_row_block() * QR + 2 * g;
+  const long long rb = q_row_block(STATS ? 1 : 0);
+  q1k_c2r_load(halfT, lds, rb, t, g, j, a);
+  const long long ra = rb * QR + 2 * g;
   constexpr float sc = 1.0f / Q1K;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -1159,16 +1190,19 @@ int scxr_cconv_apply(const scxr_cconv_plan* plan, const float* x, float* y, int3
   return convolve(plan->p, x, y, adjoint != 0, pred, as_stream(stream));
 }
 
-int scxr_cconv_apgm_grad(const scxr_cconv_plan* plan, const float* x, const float* gs,
-                         const float* Yh, double* obj, float* gx, const float* v,
-                         const float* aty, float c, float b, float* u, const int32_t* pred,
-                         void* stream) {
+}  // extern "C"
+
+// The three kernels of one fused gradient step; st (nullable) adds the
+// statistics row to the last of them.
+static int apgm_grad(const scxr_cconv_plan* plan, const float* x, const float* gs,
+                     const float* Yh, double* obj, float* gx, const float* v, const float* aty,
+                     float c, float b, float* u, const ApgmStats* st, const int32_t* pred,
+                     hipStream_t s) {
   SCXR_REQUIRE(plan && x && gs && gx && v && aty && u && (!obj || Yh), SCXR_ERR_ARG,
                "null argument to cconv_apgm_grad");
   const Plan& p = plan->p;
   SCXR_REQUIRE(q1k_path(p) && p.batch == 1, SCXR_ERR_UNSUPPORTED,
                "fused APGM gradient needs a single 1024 x 1024 real convolution plan");
-  const hipStream_t s = as_stream(stream);
   constexpr int H1 = Q1K / 2 + 1;
   float2* half = p.work;
   hipLaunchKernelGGL(q1k_r2c_kernel, dim3(Q1K / QR), dim3(2 * QT), 0, s, x, half, p.ax[1].tw,
@@ -1177,10 +1211,38 @@ int scxr_cconv_apgm_grad(const scxr_cconv_plan* plan, const float* x, const floa
   hipLaunchKernelGGL(q1k_cols_gram_kernel, dim3(H1), dim3(QT), 0, s, half, p.hs, gs,
                      reinterpret_cast<const float2*>(Yh), obj, p.ax[0].tw, pred);
   SCXR_LAUNCHED("fft gram columns (1024)");
-  hipLaunchKernelGGL(q1k_c2r_step_kernel, dim3(Q1K / QR), dim3(2 * QT), 0, s, half, gx, v, aty,
-                     1.0f + c, c, b, u, p.ax[1].tw, pred);
-  SCXR_LAUNCHED("fft c2r gradient step (1024)");
+  if (st) {
+    hipLaunchKernelGGL(q1k_c2r_step_kernel<true>, dim3(Q1K / QR + 1), dim3(2 * QT), 0, s, half,
+                       gx, v, aty, 1.0f + c, c, b, u, p.ax[1].tw, *st, pred);
+    SCXR_LAUNCHED("fft c2r gradient step + statistics row (1024)");
+  } else {
+    hipLaunchKernelGGL(q1k_c2r_step_kernel<false>, dim3(Q1K / QR), dim3(2 * QT), 0, s, half, gx,
+                       v, aty, 1.0f + c, c, b, u, p.ax[1].tw, ApgmStats{}, pred);
+    SCXR_LAUNCHED("fft c2r gradient step (1024)");
+  }
   return SCXR_OK;
+}
+
+extern "C" {
+
+int scxr_cconv_apgm_grad(const scxr_cconv_plan* plan, const float* x, const float* gs,
+                         const float* Yh, double* obj, float* gx, const float* v,
+                         const float* aty, float c, float b, float* u, const int32_t* pred,
+                         void* stream) {
+  return apgm_grad(plan, x, gs, Yh, obj, gx, v, aty, c, b, u, nullptr, pred, as_stream(stream));
+}
+
+int scxr_cconv_apgm_grad_itstat(const scxr_cconv_plan* plan, const float* x, const float* gs,
+                                const float* Yh, double* obj, float* gx, const float* v,
+                                const float* aty, float c, float b, float* u, double obj_scale,
+                                const double* tv, int32_t ntv, double lam, const double* res,
+                                int32_t nres, double* hist, int32_t* counter, int32_t hist_cap,
+                                const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(obj && Yh && tv && res && hist && counter && ntv > 0 && nres > 0 && hist_cap > 0,
+               SCXR_ERR_ARG, "bad cconv_apgm_grad_itstat arguments");
+  const ApgmStats st{obj, Q1K / 2 + 1, obj_scale, tv, ntv, lam, res, nres, hist, counter,
+                     hist_cap};
+  return apgm_grad(plan, x, gs, Yh, obj, gx, v, aty, c, b, u, &st, pred, as_stream(stream));
 }
 
 int scxr_fftn(const scxr_cconv_plan* plan, const float* x, int32_t x_real, float* out,

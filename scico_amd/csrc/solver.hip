@@ -16,6 +16,7 @@
 //                     predicated on flags[k].
 #include <cmath>
 
+#include "apgm_stats.h"
 #include "common.h"
 #include "fd_impl.h"
 #include "solver_impl.h"
@@ -585,55 +586,15 @@ __global__ __launch_bounds__(256) void admm_itstat_kernel(
                    counter, hist_cap);
 }
 
-// One AcceleratedPGM statistics row: hist[row] = {obj_scale * sum(obj) +
-// lam * sum(tv), sqrt(sum(res))} (Objective f + g and Residual,
-// scico/optimize/_pgm.py:151-156, 242-248), row from the device counter.
-// One workgroup of 1024 threads; each thread's share of a partial array is
-// loaded eight at a time before it is added, so the three sums cost about
-// one memory latency each instead of one per element.
+// One AcceleratedPGM statistics row (apgm_stats_block), reduced by one
+// workgroup of 1024 threads.
 constexpr int ITS_T = 1024;
-__device__ __forceinline__ double its_share(const double* __restrict__ p, int n) {
-  double v = 0.0;
-  for (int e0 = 0; e0 < n; e0 += 8 * ITS_T) {
-    double b[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int e = e0 + static_cast<int>(threadIdx.x) + ITS_T * i;
-      b[i] = e < n ? p[e] : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v += b[i];
-  }
-  return v;
-}
-
 __global__ __launch_bounds__(ITS_T) void apgm_itstat_kernel(
     const double* __restrict__ obj, int nobj, double obj_scale, const double* __restrict__ tv,
     int ntv, double lam, const double* __restrict__ res, int nres, double* __restrict__ hist,
     int32_t* __restrict__ counter, int hist_cap) {
-  __shared__ double sh[3][ITS_T / 64];
-  double v[3] = {its_share(obj, nobj), its_share(tv, ntv), its_share(res, nres)};
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
-    if (lane == 0) sh[q][wave] = v[q];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double so = 0.0, st = 0.0, sr = 0.0;
-    for (int w = 0; w < ITS_T / 64; ++w) {
-      so += sh[0][w];
-      st += sh[1][w];
-      sr += sh[2][w];
-    }
-    const int row = counter[0];
-    if (row < hist_cap) {
-      hist[2 * row] = obj_scale * so + lam * st;
-      hist[2 * row + 1] = sqrt(sr);
-    }
-    counter[0] = row + 1;
-  }
+  apgm_stats_block<ITS_T>(obj, nobj, obj_scale, tv, ntv, lam, res, nres, hist, counter,
+                          hist_cap);
 }
 
 // ---------------------------------------------------------------- TV --

@@ -283,8 +283,7 @@ class _FusedDeconv:
         s = dev.stream()
         call("scxr_tv_prox_apgm", self.M, self.N, P(self.u), P(self.x), P(self.v), tau, self.iso,
              self.circ, c, P(self.ws_res), P(self.ws_tv), None, s)
-        call("scxr_cconv_apgm_grad", self.A.plan, P(self.x), P(self.gs), P(self.Yh),
-             P(self.ws_obj), P(self.gx), P(self.v), P(self.aty), c, b, P(self.u), None, s)
-        call("scxr_apgm_itstat", P(self.ws_obj), self.n_obj, self.obj_scale, P(self.ws_tv),
-             self.n_tv, self.lam, P(self.ws_res), self.n_res, P(self.hist), P(self.counter),
-             self.hist_cap, s)
+        call("scxr_cconv_apgm_grad_itstat", self.A.plan, P(self.x), P(self.gs), P(self.Yh),
+             P(self.ws_obj), P(self.gx), P(self.v), P(self.aty), c, b, P(self.u), self.obj_scale,
+             P(self.ws_tv), self.n_tv, self.lam, P(self.ws_res), self.n_res, P(self.hist),
+             P(self.counter), self.hist_cap, None, s)
