@@ -51,6 +51,7 @@ struct Plan {
   int cpr = 32;                            // forward candidates per row
   int spg = 2;                             // forward slots per bin group
   bool vol_direct = false, proj_direct = false;
+  bool adj_staged = false;                 // adjoint rows staged in LDS (adj_slice_staged_kernel)
   float* mats = nullptr;   // (V, 2, 4) float32
   float4* geom = nullptr;  // slice mode: (a, b, t, 0) in-plane row per view
   float* volc = nullptr;   // canonical volume workspace (P Q rows of 4 SQ floats)
@@ -352,6 +353,131 @@ __global__ __launch_bounds__(256) void adj_slice_kernel(const float4* __restrict
         ucur = u;
         acc[i] = fma4(cr, yb, fma4(cl, ya, acc[i]));
       }
+    }
+  }
+  if (!qok) return;
+#pragma unroll
+  for (int i = 0; i < AR; ++i) {
+    const int p = p0 + (i >> 2), q = q0 + (i & 3);
+    if (p < P && q < Q) vol[(static_cast<size_t>(p) * Q + q) * SQ + quad] = acc[i];
+  }
+}
+
+// Copy one 64-quad row of slices (1 KiB per wave) from global memory into
+// LDS at dst without passing through registers (lane i writes dst + 16 i).
+__device__ __forceinline__ void lds_row_load(__amdgpu_buffer_rsrc_t r, float4* dst, unsigned voff,
+                                             int soff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 16,
+                                           voff, soff, 0, 0);
+}
+// Adjoint with the detector rows staged in LDS.  The workgroup's four
+// waves cover a 4 x 8 pixel tile; for each view the tile's bins lie
+// between those of its corner pixels (the projected centre is monotone in
+// p and q, in float32 too), a range of at most ceil(3 |a| + 7 |b|) + 2
+// rows.  That range is copied into LDS once per view (buffer loads that
+// write LDS directly, one view ahead), and each wave reads its pixels'
+// rows from there with the same reuse and accumulation order as
+// adj_slice_kernel, so the results are bitwise the same; the texture data
+// path then moves each row once per workgroup instead of once per wave.
+// The host takes this kernel when every view's range fits ADJ_NW rows.
+constexpr int ADJ_NW = 12;  // staged detector rows per view
+
+__global__ __launch_bounds__(256) void adj_slice_staged_kernel(
+    const float4* __restrict__ proj, float4* __restrict__ vol, const float4* __restrict__ geom,
+    int P, int Q, int SQ, int U, int V, const int32_t* __restrict__ pred) {
+  if (pred_off(pred)) return;
+  __shared__ float4 stage[2][ADJ_NW][64];
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int pt = blockIdx.y * 4, qt = blockIdx.x * 8;  // tile origin
+  const int p0 = pt + 2 * (w >> 1), q0 = qt + 4 * (w & 1);
+  const int quad = blockIdx.z * 64 + lane;
+  const bool qok = quad < SQ;
+  const unsigned qoff = (qok ? quad : 0) * 16u;
+  const int rowb = SQ * 16;
+  const __amdgpu_buffer_rsrc_t yr = make_rsrc(proj, V * U * rowb);
+  const int r = lane & (AR - 1), jv = lane / AR;
+  const int pp = p0 + (r >> 2), qq = q0 + (r & 3);
+  const bool pok = pp < P && qq < Q;
+
+  // Staged range (first row, row count) of views c0 .. c0 + 15: lane
+  // 4 k + corner computes corner `corner` of view c0 + k; a quad reduction
+  // leaves the range in lane 4 k.
+  auto ranges = [&](int c0, int& lo, int& n) {
+    const int vv = min(c0 + (lane >> 2), V - 1), c = lane & 3;
+    const float4 g = geom[vv];
+    const int ul = slice_tap(g.x, g.y, g.z, pt + 3 * (c >> 1), qt + 7 * (c & 1), U).ul;
+    int mn = ul, mx = ul;
+    mn = min(mn, __builtin_amdgcn_mov_dpp(mn, 0xB1, 0xF, 0xF, true));  // quad_perm 1 0 3 2
+    mx = max(mx, __builtin_amdgcn_mov_dpp(mx, 0xB1, 0xF, 0xF, true));
+    mn = min(mn, __builtin_amdgcn_mov_dpp(mn, 0x4E, 0xF, 0xF, true));  // quad_perm 2 3 0 1
+    mx = max(mx, __builtin_amdgcn_mov_dpp(mx, 0x4E, 0xF, 0xF, true));
+    lo = min(max(mn, 0), U - 1);
+    n = min(max(mx + 1, 0), U - 1) - lo + 1;
+  };
+  int lo_c, n_c, lo_n = 0, n_n = 0;  // ranges of the current and the next 16 views
+  ranges(0, lo_c, n_c);
+  if (V > 16) ranges(16, lo_n, n_n);
+  // stage view v's rows into buffer b
+  auto stage_view = [&](int v, int lo, int n, int b) {
+    for (int k = w; k < n; k += 4)
+      lds_row_load(yr, &stage[b][k][0], qoff, (v * U + lo + k) * rowb);
+  };
+
+  float4 acc[AR];
+#pragma unroll
+  for (int i = 0; i < AR; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  stage_view(0, rl(lo_c, 0), rl(n_c, 0), 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int ul = 0;
+  float wl = 0.0f, wr = 0.0f;
+#pragma unroll 1
+  for (int v = 0; v < V; ++v) {
+    const int j = v & 7;
+    if (j == 0) {  // taps of the wave's 8 pixels for views v .. v + 7
+      const int vv = v + jv;
+      ul = 0;
+      wl = wr = 0.0f;
+      if (vv < V) {
+        const float4 g = geom[vv];
+        const Tap tp = slice_tap(g.x, g.y, g.z, pp, qq, U);
+        ul = tp.ul;
+        wl = pok ? tp.wl : 0.0f;
+        wr = pok ? tp.wr : 0.0f;
+      }
+    }
+    // this view's range, then the next view's, from the 16-view range sets
+    const int lo = rl(lo_c, 4 * (v & 15));
+    if (v + 1 < V) {
+      const int vn = v + 1, kn = 4 * (vn & 15);
+      const bool nxt_set = (vn & 15) == 0;
+      const int lon = nxt_set ? rl(lo_n, kn) : rl(lo_c, kn);
+      const int nn = nxt_set ? rl(n_n, kn) : rl(n_c, kn);
+      stage_view(vn, lon, nn, vn & 1);
+    }
+    // both rows of all eight pixels first (no reuse branches: LDS reads
+    // are cheap), then the accumulation in adj_slice_kernel's order
+    const float4* sb = &stage[v & 1][0][0] + lane;
+    float4 ya[AR], yb[AR];
+#pragma unroll
+    for (int i = 0; i < AR; ++i) {
+      const int u = rl(ul, j * AR + i);
+      ya[i] = sb[(min(max(u, 0), U - 1) - lo) * 64];
+      yb[i] = sb[(min(max(u + 1, 0), U - 1) - lo) * 64];
+    }
+#pragma unroll
+    for (int i = 0; i < AR; ++i) {
+      const int L = j * AR + i;
+      const float cl = rlf(wl, L), cr = rlf(wr, L);
+      acc[i] = fma4(cr, yb[i], fma4(cl, ya[i], acc[i]));
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if ((v & 15) == 15) {  // the next 16 views become current
+      lo_c = lo_n;
+      n_c = n_n;
+      if (v + 17 < V) ranges(v + 17, lo_n, n_n);
     }
   }
   if (!qok) return;
@@ -728,6 +854,18 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
     std::vector<float4> g(V);
     for (int v = 0; v < V; ++v)
       g[v] = make_float4(mat(m, v, e, p.pax), mat(m, v, e, p.qax), mat(m, v, e, 3), 0.0f);
+    // Staged adjoint: a 4 x 8 tile's rows per view, at most
+    // ceil(3 |a| + 7 |b| + rounding) + 2 (the corner bins' spread plus the
+    // right-hand tap), must fit ADJ_NW.  The rounding margin covers the
+    // float32 evaluation of the corner centres (a few ulps of |P|).
+    p.adj_staged = true;
+    for (int v = 0; v < V; ++v) {
+      const double a = std::fabs(static_cast<double>(g[v].x)),
+                   b = std::fabs(static_cast<double>(g[v].y));
+      const double pmax = a * (p.P + 1) + b * (p.Q + 1) + std::fabs(static_cast<double>(g[v].z));
+      const double spread = 3.0 * a + 7.0 * b + 1e-3 + 1e-6 * (pmax + 2.0);
+      if (std::ceil(spread) + 2.0 > ADJ_NW) p.adj_staged = false;
+    }
     rc = dev_alloc(reinterpret_cast<void**>(&p.geom), V * sizeof(float4));
     if (rc) return rc;
     SCXR_HIP(hipMemcpy(p.geom, g.data(), V * sizeof(float4), hipMemcpyHostToDevice));
@@ -886,9 +1024,9 @@ int adjoint(const Plan& p, const float* proj, float* vol, const int32_t* pred, h
   }
   float* vc = p.vol_direct ? vol : p.volc;
   dim3 grid(ceil_div(p.Q, 8), ceil_div(p.P, 4), ceil_div(p.SQ, 64));
-  hipLaunchKernelGGL(adj_slice_kernel, grid, dim3(256), 0, s,
-                     reinterpret_cast<const float4*>(yc), reinterpret_cast<float4*>(vc), p.geom,
-                     p.P, p.Q, p.SQ, p.U, p.V, pred);
+  hipLaunchKernelGGL(p.adj_staged ? adj_slice_staged_kernel : adj_slice_kernel, grid, dim3(256), 0,
+                     s, reinterpret_cast<const float4*>(yc), reinterpret_cast<float4*>(vc),
+                     p.geom, p.P, p.Q, p.SQ, p.U, p.V, pred);
   SCXR_LAUNCHED("xr3d slice adjoint");
   if (!p.vol_direct) return canon_to_vol(p, vc, vol, pred, s);
   return SCXR_OK;
