@@ -123,7 +123,14 @@ def _solver3d(M, det, y, N, lam, rho, iters, fused=True, x0=None, iso=False):
         admm_mod._FusedTV.enabled = old
 
 
-def _noisy(A, seed=5):
+# Whole iterations are held to twice the largest drift of the oracle runs
+# whose projector outputs carry +-1 ulp noise, over JITTER_SEEDS (adjoint
+# noise seeded seed + 10); F32_RES absorbs floors that round to zero.
+JITTER_SEEDS = (1, 2, 3, 4, 5)
+F32_RES = 1e-6
+
+
+def _noisy(A, seed):
     rng = np.random.default_rng(seed)
     eps = f32(6e-8)
     return lambda v: (lambda r: (r * (1 + eps * rng.choice([-1, 0, 1], r.shape))).astype(f32))(A(v))
@@ -137,10 +144,12 @@ def test_admm_tv_3d_one_step(gpu, iso):
                               isotropic=iso, state=True)
     xr, hr, _, _ = S.admm_tv(y, A, AT, lam, rho, x1, 1, cg_maxiter=10, isotropic=iso, z0=z1,
                              u0=u1, state=True)
-    xn, hn, _, _ = S.admm_tv(y, _noisy(A), _noisy(AT, 6), lam, rho, x1, 1, cg_maxiter=10,
-                             isotropic=iso, z0=z1, u0=u1, state=True)
-    floor = rel_l2(xn, xr)
-    ofloor = abs(hn[0][1] - hr[0][1]) / hr[0][1]
+    floor = ofloor = 0.0
+    for seed in JITTER_SEEDS:
+        xn, hn, _, _ = S.admm_tv(y, _noisy(A, seed), _noisy(AT, seed + 10), lam, rho, x1, 1,
+                                 cg_maxiter=10, isotropic=iso, z0=z1, u0=u1, state=True)
+        floor = max(floor, rel_l2(xn, xr))
+        ofloor = max(ofloor, abs(hn[0][1] - hr[0][1]) / hr[0][1])
     for fused in (True, False):
         s = _solver3d(M, det, y, N, lam, rho, 1, fused, x0=x1, iso=iso)
         s.z_list, s.u_list, s.z_list_old = [dev(z1)], [dev(u1)], [dev(z1)]
@@ -149,10 +158,10 @@ def test_admm_tv_3d_one_step(gpu, iso):
         h = s.history(transpose=True)
         ex = rel_l2(x, xr)
         eo = abs(h.Objective[0] - hr[0][1]) / hr[0][1]
-        print(f"  3D ADMM step iso={iso} fused={fused}: x {ex:.2e} (floor {floor:.2e}), "
+        print(f"  3D ADMM step iso={iso} fused={fused}: x {ex:.2e} (largest floor {floor:.2e}), "
               f"objective {eo:.2e} (floor {ofloor:.2e})", flush=True)
-        assert ex <= 10 * floor + 1e-5
-        assert eo <= 10 * ofloor + 1e-5
+        assert ex <= 2 * floor + F32_RES
+        assert eo <= 2 * ofloor + F32_RES
         assert h.CG_It[0] == hr[0][4]
 
 
@@ -161,9 +170,11 @@ def test_admm_tv_3d_multi_step(gpu):
     M, det, vol, y, A, AT = _problem3d(N)
     x0 = np.zeros((N, N, N), f32)
     xr, hr = S.admm_tv(y, A, AT, lam, rho, x0, it, cg_maxiter=10, isotropic=False)
-    xn, _ = S.admm_tv(y, _noisy(A), _noisy(AT, 6), lam, rho, x0, it, cg_maxiter=10,
-                      isotropic=False)
-    floor = rel_l2(xn, xr)
+    floor = 0.0
+    for seed in JITTER_SEEDS:
+        xn, _ = S.admm_tv(y, _noisy(A, seed), _noisy(AT, seed + 10), lam, rho, x0, it,
+                          cg_maxiter=10, isotropic=False)
+        floor = max(floor, rel_l2(xn, xr))
     s = _solver3d(M, det, y, N, lam, rho, it)
     x = s.solve().cpu().numpy()
     h = s.history(transpose=True)
@@ -171,7 +182,7 @@ def test_admm_tv_3d_multi_step(gpu):
     e = rel_l2(x, xr)
     print(f"  3D ADMM x{it}: vs oracle {e:.2e} (floor {floor:.2e}); SNR {snr(x):.3f} vs "
           f"{snr(xr):.3f} dB", flush=True)
-    assert e <= max(10 * floor, 1e-5)
+    assert e <= 2 * floor + F32_RES
     assert abs(snr(x) - snr(xr)) < 0.1
     assert list(h.CG_It) == [r[4] for r in hr]
 

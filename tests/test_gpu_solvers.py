@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import cpu
 from oracle import solvers as S
 from oracle import xray as X
 from scico_amd import functional, linop, loss, phantom
@@ -122,19 +123,28 @@ def _snr(x, ref):
     return 10 * np.log10(np.var(ref) / np.mean((np.asarray(x, np.float64) - ref) ** 2))
 
 
-def _noisy_projectors(g, seed=5):
-    """Oracle projectors whose outputs carry +-1 ulp relative noise: a model
-    of an equally valid float32 implementation with another summation
-    order, used to calibrate how far two such implementations drift."""
+# Whole ADMM iterations (10-step CG on an ill-conditioned system, stopped
+# by a tolerance) amplify last-bit differences, so whole-iteration results
+# are held to twice the largest drift of other valid float32 runs from the
+# oracle: JITTER_SEEDS runs whose projector outputs carry +-1 ulp of noise.
+# Their drifts differ by up to 50x between seeds on these problems, hence
+# the maximum over five.  F32_RES absorbs floors that round to zero.
+JITTER_SEEDS = (1, 2, 3, 4, 5)
+F32_RES = 1e-6
+
+
+def _noisy_projectors(g, seed):
+    """Projectors of the C restatement (forward bitwise the NumPy oracle's,
+    adjoint in another summation order) whose outputs carry +-1 ulp
+    relative noise: a model of an equally valid float32 implementation,
+    used to calibrate how far two such implementations drift."""
     rng = np.random.default_rng(seed)
     eps = np.float32(6e-8)
 
     def jitter(r):
         return (r * (1 + eps * rng.choice([-1, 0, 1], r.shape))).astype(np.float32)
 
-    A = lambda v: jitter(X.project(v, g["x0"], g["dx"], g["y0"], g["ny"], g["angles"]))
-    AT = lambda v: jitter(X.back_project(v, g["x0"], g["dx"], g["nx"], g["y0"], g["angles"]))
-    return A, AT
+    return (lambda v: jitter(cpu.project(v, g))), (lambda v: jitter(cpu.back_project(v, g)))
 
 
 @pytest.mark.parametrize("N,A_,D,circular,iso", [(48, 60, 68, False, True),
@@ -143,21 +153,22 @@ def _noisy_projectors(g, seed=5):
                                                  (256, 180, 363, False, True)])
 def test_admm_tv_one_step(gpu, N, A_, D, circular, iso):
     """One ADMM iteration from a shared non-trivial state, fused and generic
-    paths.  A whole iteration (10-step CG on an ill-conditioned system)
-    amplifies last-bit differences, so the bar is 10x the spread of the
-    oracle against itself with +-1 ulp noise on every projector output; the
-    individual fused operations are held to 1e-5 in test_gpu_fused.py."""
+    paths, within twice the largest drift of the JITTER_SEEDS noisy oracle
+    runs (x and objective) and with the same CG step count; the individual
+    fused operations are held to 1e-5 in test_gpu_fused.py."""
     ang, g, xgt, y = _problem(N, A_, D, ne=10 if N > 100 else 8)
     lam, rho = 2e-2, 1.0
     x1, _, z1, u1 = _oracle_admm(g, y, lam, rho, 2 if N < 100 else 1, 10, circular, iso,
                                  state=True)
     xr, hr, zr, ur = _oracle_admm(g, y, lam, rho, 1, 10, circular, iso, x0=x1, z0=z1, u0=u1,
                                   state=True)
-    An, ATn = _noisy_projectors(g)
-    xn, hn, _, _ = S.admm_tv(y, An, ATn, lam, rho, x1, 1, cg_tol=1e-4, cg_maxiter=10,
-                             circular=circular, isotropic=iso, z0=z1, u0=u1, state=True)
-    floor = rel_l2(xn, xr)
-    ofloor = abs(hn[0][1] - hr[0][1]) / hr[0][1]
+    floor = ofloor = 0.0
+    for seed in JITTER_SEEDS:
+        An, ATn = _noisy_projectors(g, seed)
+        xn, hn, _, _ = S.admm_tv(y, An, ATn, lam, rho, x1, 1, cg_tol=1e-4, cg_maxiter=10,
+                                 circular=circular, isotropic=iso, z0=z1, u0=u1, state=True)
+        floor = max(floor, rel_l2(xn, xr))
+        ofloor = max(ofloor, abs(hn[0][1] - hr[0][1]) / hr[0][1])
     for fused in (True, False) if N < 100 else (True,):
         s = _gpu_solver(ang, y, N, D, lam, rho, 1, 10, circular, iso, fused, x0=x1)
         s.z_list, s.u_list, s.z_list_old = [dev(z1)], [dev(u1)], [dev(z1)]
@@ -167,32 +178,34 @@ def test_admm_tv_one_step(gpu, N, A_, D, circular, iso):
         h = s.history(transpose=True)
         eo = abs(h.Objective[0] - hr[0][1]) / hr[0][1]
         print(f"  one ADMM step {N}^2 circ={circular} iso={iso} fused={fused}: x {ex:.2e} "
-              f"(ulp-noise floor {floor:.2e}); objective {eo:.2e} (floor {ofloor:.2e})",
+              f"(largest ulp-noise floor {floor:.2e}); objective {eo:.2e} (floor {ofloor:.2e})",
               flush=True)
-        assert ex <= 10 * floor + 1e-5
-        assert eo <= 10 * ofloor + 1e-5
+        assert ex <= 2 * floor + F32_RES
+        assert eo <= 2 * ofloor + F32_RES
         assert h.CG_It[0] == hr[0][4]
 
 
 def test_admm_tv_multi_step_noise_floor(gpu):
-    """Six iterations: GPU vs oracle within 10x the oracle's drift under +-1
-    ulp projector noise; reconstruction SNR within the SNR drift of that
-    noisy oracle run (0.26 dB on this problem), at least 0.1 dB."""
+    """Six iterations: GPU vs oracle within twice the largest drift of the
+    JITTER_SEEDS noisy oracle runs; reconstruction SNR within twice their
+    largest SNR drift, at least 0.1 dB."""
     N, A_, D = 48, 60, 68
     ang, g, xgt, y = _problem(N, A_, D)
     xr, hr = _oracle_admm(g, y, 2e-2, 1.0, 6, 10)
-    An, ATn = _noisy_projectors(g)
-    xn, hn = S.admm_tv(y, An, ATn, 2e-2, 1.0, np.zeros((N, N), np.float32), 6, cg_tol=1e-4,
-                       cg_maxiter=10)
-    floor = rel_l2(xn, xr)
+    floor = snr_floor = 0.0
+    for seed in JITTER_SEEDS:
+        An, ATn = _noisy_projectors(g, seed)
+        xn, hn = S.admm_tv(y, An, ATn, 2e-2, 1.0, np.zeros((N, N), np.float32), 6, cg_tol=1e-4,
+                           cg_maxiter=10)
+        floor = max(floor, rel_l2(xn, xr))
+        snr_floor = max(snr_floor, abs(_snr(xn, xgt) - _snr(xr, xgt)))
     xf, hf = _gpu_admm(ang, y, N, D, 2e-2, 1.0, 6, 10)
     e = rel_l2(xf, xr)
-    snr_floor = abs(_snr(xn, xgt) - _snr(xr, xgt))
-    print(f"  6-step ADMM: GPU vs oracle {e:.2e}; ulp-noise floor {floor:.2e}; "
-          f"SNR gpu {_snr(xf, xgt):.4f} oracle {_snr(xr, xgt):.4f} "
-          f"noisy oracle {_snr(xn, xgt):.4f} dB", flush=True)
-    assert e <= max(10 * floor, 1e-5)
-    assert abs(_snr(xf, xgt) - _snr(xr, xgt)) <= max(snr_floor, 0.1)
+    print(f"  6-step ADMM: GPU vs oracle {e:.2e}; largest ulp-noise floor {floor:.2e}; "
+          f"SNR gpu {_snr(xf, xgt):.4f} oracle {_snr(xr, xgt):.4f} dB (largest noisy-run "
+          f"drift {snr_floor:.4f} dB)", flush=True)
+    assert e <= 2 * floor + F32_RES
+    assert abs(_snr(xf, xgt) - _snr(xr, xgt)) <= max(2 * snr_floor, 0.1)
     assert list(hf.CG_It) == [h[4] for h in hr]
 
 
