@@ -451,8 +451,12 @@ int scxr_batch_admm_itstat(const scxr_xr2d_batch* plan, const double* res, const
  * Geometries where one detector axis follows one volume axis exactly
  * (single rotation axis, e.g. matrices_from_euler_angles with seq "Y")
  * run in slice mode: slice-parallel kernels, deterministic, no atomics.
- * Others run in general mode (forward with float atomics).  allow_slice = 0
- * forces general mode.
+ * Others run in general mode (gathers in a fixed order, also
+ * deterministic).  allow_slice is a set of flags: bit 0 clear forces
+ * general mode; bit 1 set keeps the slice adjoint that reads the detector
+ * rows from global memory instead of staging them in LDS (the two give
+ * bitwise-equal results; the staged one is faster).  Pass 1 for the
+ * default.
  */
 typedef struct scxr_xr3d_plan scxr_xr3d_plan;
 
@@ -460,9 +464,10 @@ int scxr_xr3d_plan_create(int32_t X, int32_t Y, int32_t Z, int32_t V,
                           const float* matrices /* host, V x 2 x 4 */, int32_t D0,
                           int32_t D1, int32_t allow_slice, scxr_xr3d_plan** out);
 int scxr_xr3d_plan_destroy(scxr_xr3d_plan* plan);
-/* out (host, 8): mode (0 general, 1 slice), slice volume axis, slice
+/* out (host, 9): mode (0 general, 1 slice), slice volume axis, slice
  * detector axis, column offset, forward candidates per row, volume used in
- * place, projections used in place, in-plane detector size. */
+ * place, projections used in place, in-plane detector size, adjoint rows
+ * staged in LDS (1) or read from global memory (0). */
 int scxr_xr3d_info(const scxr_xr3d_plan* plan, int32_t* out);
 /* proj = H vol  (_xray3d.py:110-159). */
 int scxr_xr3d_forward(const scxr_xr3d_plan* plan, const float* vol, float* proj,

@@ -804,7 +804,7 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
   std::vector<float> m(mats, mats + 8 * static_cast<size_t>(V));
   for (float f : m)
     SCXR_REQUIRE(std::isfinite(f), SCXR_ERR_GEOM, "non-finite projection matrix entry");
-  if (allow_slice) {
+  if (allow_slice & 1) {
     for (int dax = 1; dax >= 0 && p.mode != MODE_SLICE; --dax)
       for (int sax = 0; sax < 3 && p.mode != MODE_SLICE; ++sax) try_slice(p, m, sax, dax);
   }
@@ -858,8 +858,8 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
     // ceil(3 |a| + 7 |b| + rounding) + 2 (the corner bins' spread plus the
     // right-hand tap), must fit ADJ_NW.  The rounding margin covers the
     // float32 evaluation of the corner centres (a few ulps of |P|).
-    p.adj_staged = true;
-    for (int v = 0; v < V; ++v) {
+    p.adj_staged = !(allow_slice & 2);
+    for (int v = 0; v < V && p.adj_staged; ++v) {
       const double a = std::fabs(static_cast<double>(g[v].x)),
                    b = std::fabs(static_cast<double>(g[v].y));
       const double pmax = a * (p.P + 1) + b * (p.Q + 1) + std::fabs(static_cast<double>(g[v].z));
@@ -1080,6 +1080,7 @@ int scxr_xr3d_info(const scxr_xr3d_plan* plan, int32_t* out) {
   out[5] = p.vol_direct;
   out[6] = p.proj_direct;
   out[7] = p.U;
+  out[8] = p.adj_staged;
   return SCXR_OK;
 }
 

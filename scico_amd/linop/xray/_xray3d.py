@@ -55,6 +55,7 @@ class XRayTransform3D(LinearOperator):
             raise ValueError("only float32 volumes are supported")
         self._plan = None
         self._allow_slice = True
+        self._stage_adjoint = True
         super().__init__(
             input_shape=self.input_shape,
             output_shape=self.output_shape,
@@ -74,7 +75,8 @@ class XRayTransform3D(LinearOperator):
             call(
                 "scxr_xr3d_plan_create", X, Y, Z, self.output_shape[0],
                 self.matrices.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                self.det_shape[0], self.det_shape[1], int(self._allow_slice), ctypes.byref(h),
+                self.det_shape[0], self.det_shape[1],
+                int(self._allow_slice) | (0 if self._stage_adjoint else 2), ctypes.byref(h),
             )
             self._plan = h
             self._finalizer = weakref.finalize(self, lib.scxr_xr3d_plan_destroy, h)
@@ -82,12 +84,12 @@ class XRayTransform3D(LinearOperator):
 
     def info(self) -> dict:
         """Execution mode chosen for this geometry."""
-        out = np.zeros(8, dtype=np.int32)
+        out = np.zeros(9, dtype=np.int32)
         call("scxr_xr3d_info", self.plan, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         return {"mode": "slice" if out[0] == 1 else "general", "slice_axis": int(out[1]),
                 "det_axis": int(out[2]), "delta": int(out[3]), "candidates_per_row": int(out[4]),
                 "volume_in_place": bool(out[5]), "projection_in_place": bool(out[6]),
-                "in_plane_bins": int(out[7])}
+                "in_plane_bins": int(out[7]), "adjoint_staged": bool(out[8])}
 
     def project(self, im, out=None, pred=None):
         """X-ray projection (``_xray3d.py:95-100``)."""
