@@ -441,26 +441,75 @@ __global__ __launch_bounds__(256) void admm_lhs_nd_kernel(int ndim, int d0, int 
   __shared__ double sh[4];
   const NdIdx t(d0, d1, d2);
   const bool cc = circ != 0;
+  // Each thread takes elements e, e + S, e + 2 S, ... (S the grid's thread
+  // count) in that order, NW at a time: the loads of a batch are issued
+  // before any of its sums (the pass is bound by memory latency, not by
+  // arithmetic), and the coordinates advance by S with carries instead of
+  // being divided out per element.  Results and the order of the dot
+  // partial are those of one element per step.
+  constexpr int NW = 4;
+  const int64_t S = 256ll * gridDim.x;
+  int64_t e = blockIdx.x * 256ll + threadIdx.x;
+  int idx[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
+  if (e < t.n) t.split(e, idx);
+  if (S < t.n) t.split(S, sx);
   double dot = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
-    int idx[3];
-    t.split(e, idx);
-    const float xc = x[e];
-    float acc = 0.0f;
-    for (int c = 0; c < ndim; ++c) {
-      const int ax = 3 - ndim + c;
-      const int k = idx[ax], len = t.dims[ax];
-      const int kp = (k + 1 < len) ? k + 1 : 0;
-      const int km = (k >= 1) ? k - 1 : len - 1;
-      const int64_t st = t.st[ax];
-      const float g0 = fd::diff_fwd(xc, x[e + (kp - k) * st], k, len, cc);
-      const float gm = fd::diff_fwd(x[e + (km - k) * st], xc, km, len, cc);
-      const float v = fd::diff_adj(gm, g0, k, len, cc);
-      acc = (c == 0) ? v : __fadd_rn(acc, v);
+  for (; e < t.n; e += NW * S) {
+    float xc[NW], at[NW], xp[NW][3], xm[NW][3];
+    int id[NW][3];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int64_t ew = e + w * S;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) id[w][a] = idx[a];
+      // advance idx by S (sx[1] < d1, sx[2] < d2: one carry each)
+      idx[2] += sx[2];
+      if (idx[2] >= t.dims[2]) {
+        idx[2] -= t.dims[2];
+        ++idx[1];
+      }
+      idx[1] += sx[1];
+      if (idx[1] >= t.dims[1]) {
+        idx[1] -= t.dims[1];
+        ++idx[0];
+      }
+      idx[0] += sx[0];
+      if (ew < t.n) {
+        xc[w] = x[ew];
+        at[w] = ata[ew];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (c >= ndim) break;
+          const int ax = 3 - ndim + c;
+          const int k = id[w][ax], len = t.dims[ax];
+          const int kp = (k + 1 < len) ? k + 1 : 0;
+          const int km = (k >= 1) ? k - 1 : len - 1;
+          const int64_t st = t.st[ax];
+          xp[w][c] = x[ew + (kp - k) * st];
+          xm[w][c] = x[ew + (km - k) * st];
+        }
+      }
     }
-    const float o = __fadd_rn(__fmul_rn(rho, acc), ata[e]);
-    out[e] = o;
-    dot += static_cast<double>(xc) * o;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int64_t ew = e + w * S;
+      if (ew >= t.n) break;
+      float acc = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (c >= ndim) break;
+        const int ax = 3 - ndim + c;
+        const int k = id[w][ax], len = t.dims[ax];
+        const int km = (k >= 1) ? k - 1 : len - 1;
+        const float g0 = fd::diff_fwd(xc[w], xp[w][c], k, len, cc);
+        const float gm = fd::diff_fwd(xm[w][c], xc[w], km, len, cc);
+        const float v = fd::diff_adj(gm, g0, k, len, cc);
+        acc = (c == 0) ? v : __fadd_rn(acc, v);
+      }
+      const float o = __fadd_rn(__fmul_rn(rho, acc), at[w]);
+      out[ew] = o;
+      dot += static_cast<double>(xc[w]) * o;
+    }
   }
   if (dot_ws) {
     dot = block_sum_f64(dot, sh);

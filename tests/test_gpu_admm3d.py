@@ -36,7 +36,9 @@ def _shape_c(shape):
     return (ctypes.c_int32 * len(shape))(*shape)
 
 
-@pytest.mark.parametrize("shape", [(7, 9, 11), (12, 10), (33,)])
+# (64, 128, 160): more elements than one pass of the 1024-block grid, so
+# threads take several batches and the coordinates carry on every axis.
+@pytest.mark.parametrize("shape", [(7, 9, 11), (12, 10), (33,), (64, 128, 160)])
 @pytest.mark.parametrize("circular", [False, True])
 @pytest.mark.parametrize("iso", [True, False])
 def test_nd_fused_ops(gpu, shape, circular, iso):
