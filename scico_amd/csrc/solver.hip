@@ -396,6 +396,30 @@ struct NdIdx {
   }
 };
 
+// Coordinates along a grid-stride walk e, e + S, e + 2 S, ...: split once,
+// then advanced by S with one carry per axis (S split as sx, sx[1] < d1,
+// sx[2] < d2), so the loop divides nothing per element.
+struct NdWalk {
+  int idx[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
+  __device__ NdWalk(const NdIdx& t, int64_t e, int64_t S) {
+    if (e < t.n) t.split(e, idx);
+    if (S < t.n) t.split(S, sx);
+  }
+  __device__ __forceinline__ void next(const NdIdx& t) {
+    idx[2] += sx[2];
+    if (idx[2] >= t.dims[2]) {
+      idx[2] -= t.dims[2];
+      ++idx[1];
+    }
+    idx[1] += sx[1];
+    if (idx[1] >= t.dims[1]) {
+      idx[1] -= t.dims[1];
+      ++idx[0];
+    }
+    idx[0] += sx[0];
+  }
+};
+
 // (C^T w)[e] for w = a - b stacked (ndim, n) (b may be null), axis order sum.
 __device__ __forceinline__ float adj_nd(const float* __restrict__ a, const float* __restrict__ b,
                                         const NdIdx& t, int ndim, int64_t e, const int idx[3],
@@ -422,9 +446,11 @@ __global__ __launch_bounds__(256) void admm_rhs_nd_kernel(int ndim, int d0, int 
                                                           const float* __restrict__ u, float rho,
                                                           int circ, float* out) {
   const NdIdx t(d0, d1, d2);
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
-    int idx[3];
-    t.split(e, idx);
+  const int64_t S = 256ll * gridDim.x;
+  NdWalk wk(t, blockIdx.x * 256ll + threadIdx.x, S);
+#pragma unroll 4
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += S, wk.next(t)) {
+    const int* idx = wk.idx;
     const float c = __fmul_rn(rho, adj_nd(z, u, t, ndim, e, idx, circ != 0));
     out[e] = __fadd_rn(base[e], c);
   }
@@ -444,15 +470,12 @@ __global__ __launch_bounds__(256) void admm_lhs_nd_kernel(int ndim, int d0, int 
   // Each thread takes elements e, e + S, e + 2 S, ... (S the grid's thread
   // count) in that order, NW at a time: the loads of a batch are issued
   // before any of its sums (the pass is bound by memory latency, not by
-  // arithmetic), and the coordinates advance by S with carries instead of
-  // being divided out per element.  Results and the order of the dot
-  // partial are those of one element per step.
+  // arithmetic), and the coordinates advance by S (NdWalk).  Results and
+  // the order of the dot partial are those of one element per step.
   constexpr int NW = 4;
   const int64_t S = 256ll * gridDim.x;
   int64_t e = blockIdx.x * 256ll + threadIdx.x;
-  int idx[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
-  if (e < t.n) t.split(e, idx);
-  if (S < t.n) t.split(S, sx);
+  NdWalk wk(t, e, S);
   double dot = 0.0;
   for (; e < t.n; e += NW * S) {
     float xc[NW], at[NW], xp[NW][3], xm[NW][3];
@@ -461,19 +484,8 @@ __global__ __launch_bounds__(256) void admm_lhs_nd_kernel(int ndim, int d0, int 
     for (int w = 0; w < NW; ++w) {
       const int64_t ew = e + w * S;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) id[w][a] = idx[a];
-      // advance idx by S (sx[1] < d1, sx[2] < d2: one carry each)
-      idx[2] += sx[2];
-      if (idx[2] >= t.dims[2]) {
-        idx[2] -= t.dims[2];
-        ++idx[1];
-      }
-      idx[1] += sx[1];
-      if (idx[1] >= t.dims[1]) {
-        idx[1] -= t.dims[1];
-        ++idx[0];
-      }
-      idx[0] += sx[0];
+      for (int a = 0; a < 3; ++a) id[w][a] = wk.idx[a];
+      wk.next(t);
       if (ew < t.n) {
         xc[w] = x[ew];
         at[w] = ata[ew];
@@ -533,9 +545,11 @@ __global__ __launch_bounds__(256) void admm_zu_nd_kernel(int ndim, int d0, int d
   const NdIdx t(d0, d1, d2);
   const bool cc = circ != 0;
   double prml = 0.0, gsum = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
-    int idx[3];
-    t.split(e, idx);
+  const int64_t S = 256ll * gridDim.x;
+  NdWalk wk(t, blockIdx.x * 256ll + threadIdx.x, S);
+#pragma unroll 4
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += S, wk.next(t)) {
+    const int* idx = wk.idx;
     const float xc = x[e];
     float cx[3], v[3], uo[3];
     for (int c = 0; c < ndim; ++c) {
@@ -582,9 +596,11 @@ __global__ __launch_bounds__(256) void admm_dual_nd_kernel(int ndim, int d0, int
   __shared__ double sh[4];
   const NdIdx t(d0, d1, d2);
   double s = 0.0;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += 256ll * gridDim.x) {
-    int idx[3];
-    t.split(e, idx);
+  const int64_t S = 256ll * gridDim.x;
+  NdWalk wk(t, blockIdx.x * 256ll + threadIdx.x, S);
+#pragma unroll 4
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < t.n; e += S, wk.next(t)) {
+    const int* idx = wk.idx;
     const float v = __fmul_rn(rho, adj_nd(z, zold, t, ndim, e, idx, circ != 0));
     s += static_cast<double>(v) * v;
   }
