@@ -366,7 +366,7 @@ def extra_batch(steps=50, warmup=1):
         angles.append(phantom.angles(int(rng.integers(90, 721))))
     B = XRayTransform2DBatch(shapes, angles)
     t0 = time.perf_counter()
-    tuned = B.autotune(vpgs=(1, 2), grids=(0, 2, 4), orders=(0, 1, 2), reps=2)
+    tuned = B.autotune(vpgs=(1, 2), grids=(0, 2, 4), orders=(0, 1, 2), reps=2, adj_groups=(3, 2))
     t_tune = time.perf_counter() - t0
     xs = [phantom.ellipse_phantom(sh[0], 10, seed=k) for k, sh in enumerate(shapes)]
     y = B.project(B.pack_images(xs))
@@ -394,7 +394,8 @@ def extra_batch(steps=50, warmup=1):
     inter = float(sum(s[0] * s[1] * len(a) for s, a in zip(shapes, angles)))
     return {"workload": "batched_256_ragged_slices_admm_tv_cg10_fixedwork", "slices": 256,
             "interactions_per_pass": inter, "autotuned": {k: tuned[k] for k in
-                                                          ("vpg", "grid", "order")},
+                                                          ("vpg", "grid", "order",
+                                                           "adj_groups")},
             "autotune_seconds": round(t_tune, 2),
             "projector_pass_ms": round(tuned["ms"], 3),
             "projector_interactions_per_s": round(2 * inter / (tuned["ms"] * 1e-3), 1),

@@ -407,6 +407,13 @@ int scxr_xr2d_batch_offsets(const scxr_xr2d_batch* plan, int64_t* ioff /* host, 
  * (0 slice-major, 1 round robin, 2 decreasing cost); set by the autotuner. */
 int scxr_xr2d_batch_configure(scxr_xr2d_batch* plan, int32_t fwd_grid, int32_t adj_grid,
                               int32_t order);
+/* View groups per adjoint tile: 3 (default; each slice's back projection
+ * bitwise that of scxr_xr2d_adjoint) or 2 (fewer threads per tile, more
+ * tiles resident on batches of many tiles; each pixel's sum is split over
+ * the views differently, so results agree with the single-slice path to
+ * float32 rounding only).  No reference counterpart: a tuning choice of
+ * the batched launch, set by the autotuner. */
+int scxr_xr2d_batch_set_adjoint_groups(scxr_xr2d_batch* plan, int32_t groups);
 /* sino = A im for every slice; with y and res_ws also per-group partials of
  * ||y - A im||^2 (res_ws has "forward groups" doubles). */
 int scxr_xr2d_batch_forward(const scxr_xr2d_batch* plan, const float* im, float* sino,

@@ -69,6 +69,7 @@ SIGNATURES = [
     ("scxr_xr2d_batch_info", c_int32, [c_void_p, POINTER(c_int64)]),
     ("scxr_xr2d_batch_offsets", c_int32, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     ("scxr_xr2d_batch_configure", c_int32, [c_void_p, c_int32, c_int32, c_int32]),
+    ("scxr_xr2d_batch_set_adjoint_groups", c_int32, [c_void_p, c_int32]),
     ("scxr_xr2d_batch_forward", c_int32,
      [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     ("scxr_xr2d_batch_adjoint", c_int32,

@@ -74,7 +74,7 @@ class XRayTransform2DBatch:
         self._plan = None
         self._retired = []
         self.generation = 0
-        self.config = (0, 0, 0)
+        self.config = (0, 0, 0, 3)
         self._create()
 
     def _create(self):
@@ -112,9 +112,15 @@ class XRayTransform2DBatch:
     def plan(self):
         return self._plan
 
-    def configure(self, fwd_grid: int = 0, adj_grid: int = 0, order: int = 0):
+    def configure(self, fwd_grid: int = 0, adj_grid: int = 0, order: int = 0,
+                  adj_groups: int = 3):
+        """Persistent grid sizes, item order, and view groups per adjoint
+        tile (3: every slice's back projection bitwise the single-slice
+        one; 2: the views of each pixel summed in two shares, equal to it
+        to float32 rounding)."""
         call("scxr_xr2d_batch_configure", self._plan, int(fwd_grid), int(adj_grid), int(order))
-        self.config = (int(fwd_grid), int(adj_grid), int(order))
+        call("scxr_xr2d_batch_set_adjoint_groups", self._plan, int(adj_groups))
+        self.config = (int(fwd_grid), int(adj_grid), int(order), int(adj_groups))
 
     # --- packing
     def pack_images(self, images) -> torch.Tensor:
@@ -195,10 +201,12 @@ class XRayTransform2DBatch:
         return e0.elapsed_time(e1) / reps
 
     def autotune(self, vpgs=(1, 2), grids=(0, 2, 4, 8), orders=(0, 1, 2), reps: int = 3,
-                 cu_count: Optional[int] = None, verbose: bool = False):
-        """Time (views per forward group, persistent grid, item order) on
-        this batch and keep the fastest; returns the table of timings (ms
-        per forward + adjoint pass).  Grid g > 0 means g workgroups per CU."""
+                 cu_count: Optional[int] = None, verbose: bool = False, adj_groups=(3,)):
+        """Time (views per forward group, persistent grid, item order,
+        adjoint view groups) on this batch and keep the fastest; returns the
+        table of timings (ms per forward + adjoint pass).  Grid g > 0 means
+        g workgroups per CU.  Only vpg 1 with 3 adjoint groups keeps every
+        slice bitwise equal to the single-slice path."""
         if cu_count is None:
             cu = ctypes.c_int32()
             call("scxr_device_info", ctypes.byref(cu), None, None, None, 0)
@@ -218,20 +226,24 @@ class XRayTransform2DBatch:
                     continue  # vpg not supported for this detector size
             for o in orders:
                 for gmul in grids:
-                    g = gmul * cu_count
-                    self.configure(g, g, o)
-                    t = self._time(im, sino, out_s, out_i, reps)
-                    table.append({"vpg": v, "order": o, "grid": g, "ms": t})
-                    if verbose:
-                        print(f"  autotune vpg={v} order={o} grid={g}: {t:.3f} ms", flush=True)
-                    if best is None or t < best[0]:
-                        best = (t, v, g, o)
-        _, v, g, o = best
+                    for ag in adj_groups:
+                        g = gmul * cu_count
+                        self.configure(g, g, o, ag)
+                        t = self._time(im, sino, out_s, out_i, reps)
+                        table.append({"vpg": v, "order": o, "grid": g, "adj_groups": ag,
+                                      "ms": t})
+                        if verbose:
+                            print(f"  autotune vpg={v} order={o} grid={g} adj_groups={ag}: "
+                                  f"{t:.3f} ms", flush=True)
+                        if best is None or t < best[0]:
+                            best = (t, v, g, o, ag)
+        _, v, g, o, ag = best
         if self.vpg != v:
             self.vpg = v
             self._create()
-        self.configure(g, g, o)
-        self.tuned = {"vpg": v, "grid": g, "order": o, "ms": best[0], "table": table}
+        self.configure(g, g, o, ag)
+        self.tuned = {"vpg": v, "grid": g, "order": o, "adj_groups": ag, "ms": best[0],
+                      "table": table}
         return self.tuned
 
 

@@ -361,6 +361,14 @@ int scxr_xr2d_batch_configure(scxr_xr2d_batch* plan, int32_t fwd_grid, int32_t a
   return batch::configure(plan->b, fwd_grid, adj_grid, order);
 }
 
+int scxr_xr2d_batch_set_adjoint_groups(scxr_xr2d_batch* plan, int32_t groups) {
+  SCXR_REQUIRE(plan, SCXR_ERR_ARG, "null argument to batch_set_adjoint_groups");
+  SCXR_REQUIRE(groups == 2 || groups == xr2d::ADJ_VS, SCXR_ERR_ARG,
+               "adjoint view groups must be 2 or %d", xr2d::ADJ_VS);
+  plan->b.bt.avs = groups;
+  return SCXR_OK;
+}
+
 int scxr_xr2d_batch_forward(const scxr_xr2d_batch* plan, const float* im, float* sino,
                             const float* y, double* res_ws, const int32_t* pred,
                             int32_t pstride, void* stream) {

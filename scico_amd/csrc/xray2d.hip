@@ -771,10 +771,11 @@ struct __align__(16) ViewLds {
 };
 
 // One adjoint tile (tile row by, column bx; dot partial written at index
-// dot_i).  The workgroup is ADJ_VS groups of 256 threads over the same
-// pixels; group g takes the g-th contiguous share of the views, and the
-// groups' sums are added in group order at the end.
-template <int PPT, bool FR>
+// dot_i).  The workgroup is VS groups of 256 threads over the same pixels
+// (ADJ_VS for single slices; batches may take 2); group g takes the g-th
+// contiguous share of the views, and the groups' sums are added in group
+// order at the end, so VS decides each pixel's summation order.
+template <int PPT, bool FR, int VS = ADJ_VS>
 __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* __restrict__ out,
                                          const float4* __restrict__ geom,
                                          const double* __restrict__ rwt,
@@ -782,9 +783,9 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
                                          int D, int W, const AdjEpi& epi, int bx, int by,
                                          int dot_i) {
   constexpr int VPW = ADJ_VC / 4;  // views staged per wave
-  __shared__ float win_all[ADJ_VS][ADJ_VC * ADJ_WFAST];
-  __shared__ ViewLds gv_all[ADJ_VS][ADJ_VC];
-  __shared__ double red[4 * ADJ_VS];
+  __shared__ float win_all[VS][ADJ_VC * ADJ_WFAST];
+  __shared__ ViewLds gv_all[VS][ADJ_VC];
+  __shared__ double red[4 * VS];
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
   float* win = win_all[grp];
   ViewLds* gv = gv_all[grp];
@@ -808,7 +809,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
   const char* winb = reinterpret_cast<const char*>(&win_all[0][0]);
   const int gbase4 = grp * (ADJ_VC * ADJ_WFAST * 4);
   // this group's views [vb, ve); every group runs the same number of chunks
-  const int share = (A + ADJ_VS - 1) / ADJ_VS;
+  const int share = (A + VS - 1) / VS;
   const int vb = min(A, grp * share), ve = min(A, vb + share);
   const int nchunk = (share + ADJ_VC - 1) / ADJ_VC;
 
@@ -916,9 +917,9 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
     }
   }
   // add the other groups' sums, in group order, into group 0
-  if (ADJ_VS > 1) {
+  if (VS > 1) {
     float* xch = win_all[0];  // 2 PPT floats per thread of each group > 0
-    static_assert((ADJ_VS - 1) * 2 * PPT * 256 <= ADJ_VC * ADJ_WFAST,
+    static_assert((VS - 1) * 2 * PPT * 256 <= ADJ_VC * ADJ_WFAST,
                   "group exchange must fit one window buffer");
     __syncthreads();
     if (grp > 0) {
@@ -930,7 +931,7 @@ __device__ __forceinline__ void adj_tile(const float* __restrict__ sino, float* 
     }
     __syncthreads();
     if (grp == 0) {
-      for (int g2 = 1; g2 < ADJ_VS; ++g2)
+      for (int g2 = 1; g2 < VS; ++g2)
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
           acc0[q] = __fadd_rn(acc0[q], xch[((g2 - 1) * 2 * PPT + 2 * q) * 256 + t]);
@@ -1149,8 +1150,8 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_batch_kernel(BatchTabs t, 
   }
 }
 
-template <int PPT, bool FR>
-__global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4))) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
+template <int PPT, bool FR, int VS>
+__global__ __launch_bounds__(256 * VS) __attribute__((amdgpu_waves_per_eu(4))) void adj_batch_kernel(BatchTabs t, const float* __restrict__ sino,
                                                         float* __restrict__ out, BatchEpi be,
                                                         const int32_t* pred, int pstride) {
   for (int it = blockIdx.x; it < t.na; it += gridDim.x) {
@@ -1168,8 +1169,8 @@ __global__ __launch_bounds__(256 * ADJ_VS) __attribute__((amdgpu_waves_per_eu(4)
     e.z = be.z ? be.z + 2 * io : nullptr;
     e.u = be.u ? be.u + 2 * io : nullptr;
     e.dot_ws = be.dot_ws ? be.dot_ws + t.toff[s] : nullptr;
-    adj_tile<PPT, FR>(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y, sh.z,
-                  sh.w, me.x, e, w.y % me.z, w.y / me.z, w.y);
+    adj_tile<PPT, FR, VS>(sino + so, out + io, t.geom + vo, t.rw + vo, t.clo + vo, sh.x, sh.y,
+                          sh.z, sh.w, me.x, e, w.y % me.z, w.y / me.z, w.y);
     __syncthreads();
   }
 }
@@ -1217,9 +1218,15 @@ int batch_forward(const BatchTabs& t, int V, int Dmax, int grid, const float* im
 int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
                   const BatchEpi& epi, const int32_t* pred, int pstride, hipStream_t s) {
   const unsigned g = grid > 0 ? std::min(grid, t.na) : t.na;
-#define SCXR_ADJB(PP, FF)                                                                   \
-  hipLaunchKernelGGL((adj_batch_kernel<PP, FF>), dim3(g), dim3(256 * ADJ_VS), 0, s, t, sino, im, \
-                     epi, pred, pstride)
+#define SCXR_ADJB(PP, FF)                                                                  \
+  do {                                                                                     \
+    if (t.avs == 2)                                                                        \
+      hipLaunchKernelGGL((adj_batch_kernel<PP, FF, 2>), dim3(g), dim3(512), 0, s, t, sino, im, \
+                         epi, pred, pstride);                                              \
+    else                                                                                   \
+      hipLaunchKernelGGL((adj_batch_kernel<PP, FF, ADJ_VS>), dim3(g), dim3(256 * ADJ_VS), 0, s, \
+                         t, sino, im, epi, pred, pstride);                                 \
+  } while (0)
   if (t.ppt == 2) {
     if (t.fract)
       SCXR_ADJB(2, true);

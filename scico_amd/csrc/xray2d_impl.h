@@ -126,6 +126,7 @@ struct BatchTabs {
   int ppt = 1;                    // adjoint pixels per thread (all slices)
   int fract = 0;                  // 1: Px >= 1 for every pixel of every slice
   int ch = 2;                     // forward chains per wave (all slices)
+  int avs = ADJ_VS;               // adjoint view groups per tile (2 or ADJ_VS)
 };
 
 // Epilogue arguments of the batched adjoint (packed vectors; z, u are
