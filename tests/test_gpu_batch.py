@@ -222,3 +222,31 @@ def test_full_batch_256_slices_two_adjoint_groups(gpu):
     assert worst <= 1e-6
     with pytest.raises(Exception):
         B.configure(0, 0, 2, adj_groups=4)
+
+
+def test_batch_admm_256_slices_config_length(gpu):
+    """The batched configuration at its own length: 256 ragged slices (N in
+    {128, 192, 256, 384, 512}, 90..720 views, det_count = ceil(sqrt(2) N),
+    ellipse phantoms, 1 % noise), 50 ADMM-TV iterations (lam 2e-2, rho 1,
+    CG 10 steps, tol 1e-4) in one batched solver with the bitwise
+    configuration (one view per forward group, three view groups per
+    adjoint tile); 12 slices drawn at random equal single-slice fused ADMM
+    runs bit for bit, image and every statistics row."""
+    shapes, angles = _ragged(256, [128, 192, 256, 384, 512], 90, 720, 8)
+    ys = _problem(shapes, angles, seed=20)
+    lam, rho, iters = 2e-2, 1.0, 50
+    B = XRayTransform2DBatch(shapes, angles)
+    B.configure(0, 0, 2)
+    bs = ADMMTVBatch(B, ys, lam, rho, cg_maxiter=10, cg_tol=1e-4, hist_cap=iters)
+    bs.reset()
+    for _ in range(iters):
+        bs.enqueue()
+    xs = bs.images()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(21)
+    for s in sorted(rng.choice(B.ns, 12, replace=False).tolist()):
+        xr, hr = _single_admm(shapes[s], angles[s], ys[s], lam, rho, iters, 1e-4)
+        print(f"  slice {s} {shapes[s]} views {len(angles[s])}: last statistics row "
+              f"{hr[-1].cpu().numpy()}", flush=True)
+        assert torch.equal(xs[s], xr), s
+        assert torch.equal(bs.hist[s, :iters], hr), s
