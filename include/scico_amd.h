@@ -100,6 +100,21 @@ int scxr_xr2d_weights(const scxr_xr2d_plan* plan, int32_t* inds, float* weights,
 int scxr_ramp_filter(int32_t A, int32_t D, int32_t Dout, int32_t shift, const float* taps,
                      const float* y, float* out, const int32_t* pred, void* stream);
 
+/* Forward line split (a launch choice with no counterpart in the
+ * reference): each forward group's image lines are divided over
+ * `parts` workgroups (1..8) whose partial sinograms are added in part
+ * order, which balances the CUs when there are few views.  parts = 0
+ * restores the automatic choice made at plan creation (4 for images of at
+ * least 256 x 256 pixels with fewer than two forward workgroups per CU,
+ * else 1).  The split changes the
+ * float32 summation order of a detector bin (results move by rounding,
+ * relative L2 about 1e-7); with parts = 1 the plan's forward is bitwise
+ * equal to the batched projector's for the same slice.  The effective
+ * count (always 1 for the general-geometry forward) is written to
+ * *effective when it is not null.  Calls on one plan on different streams
+ * share its workspace and must not overlap. */
+int scxr_xr2d_set_forward_split(scxr_xr2d_plan* plan, int32_t parts, int32_t* effective);
+
 /* Launch facts: out[0] forward workgroups (= partial count of
  * scxr_xr2d_forward_res), out[1] adjoint workgroups (= partial count of the
  * adjoint dot epilogue), out[2] views per forward workgroup, out[3] adjoint

@@ -57,6 +57,7 @@ SIGNATURES = [
     ("scxr_xr2d_adjoint", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_weights", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_info", c_int32, [c_void_p, c_int32_p]),
+    ("scxr_xr2d_set_forward_split", c_int32, [c_void_p, c_int32, c_int32_p]),
     ("scxr_xr2d_forward_res", c_int32,
      [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_adjoint_epi", c_int32,

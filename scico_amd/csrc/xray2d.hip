@@ -293,10 +293,57 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
     if (hipMalloc(&p.work, sizeof(float) * static_cast<size_t>(M) * N) != hipSuccess)
       return fail(SCXR_ERR_NOMEM, "hipMalloc of the transpose workspace failed");
   }
+  p.fwd_sp = 1;
+  if (set_split(p, auto_split(p)) != SCXR_OK) {
+    (void)hipGetLastError();
+    p.fwd_sp = 1;  // no workspace: one workgroup per group
+  }
+  return SCXR_OK;
+}
+
+// A forward group's lines can be split over several workgroups, whose
+// partial rows are added in a fixed order afterwards: with one workgroup
+// per view, a few hundred views leave the 256 CUs unevenly loaded.
+// Automatic choice, from a sweep of 1..8 parts on MI355X (90..720 views,
+// 128^2..512^2 pixels): 4 parts (within 10 % of the best count in every
+// case measured; 512^2 with 360 views 94 -> 65 us, with 90 views 51 ->
+// 35 us) for images of at least 256 x 256 pixels with fewer than two
+// forward groups per CU.  Smaller images keep one workgroup per group (the
+// extra launch costs more than the split saves: 128^2 with 60 views 12.5 ->
+// 14.8 us), and so do plans with more groups: at 512^2 with 720 views the
+// split would save 13 % of the forward, but its rounding took a 100-step
+// ADMM-TV run with early-stopping CG outside the drift envelope of the
+// reference-arithmetic runs at one iteration, which the unsplit projector
+// stays well inside.
+int auto_split(const Plan& p) {
+  if (p.fwd_generic || static_cast<long long>(p.M) * p.N < 65536) return 1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    return 1;
+  return p.n_groups < 2 * cus ? 4 : 1;
+}
+
+// Sets the forward split (1..FWD_SP_MAX parts; 1 for the general-geometry
+// forward), growing the partial-sinogram workspace when needed.
+int set_split(Plan& p, int parts) {
+  if (p.fwd_generic) parts = 1;
+  if (parts > p.fwd_sp_cap) {
+    float* w = nullptr;
+    if (hipMalloc(&w, sizeof(float) * parts * static_cast<size_t>(p.A) * p.D) != hipSuccess)
+      return fail(SCXR_ERR_NOMEM, "hipMalloc of the %d-part forward workspace failed", parts);
+    if (p.parts) (void)hipFree(p.parts);
+    p.parts = w;
+    p.fwd_sp_cap = parts;
+  }
+  p.fwd_sp = parts;
   return SCXR_OK;
 }
 
 void destroy(Plan& p) {
+  if (p.parts) (void)hipFree(p.parts);
+  p.parts = nullptr;
   if (p.dev_block) (void)hipFree(p.dev_block);
   if (p.work) (void)hipFree(p.work);
   p.dev_block = nullptr;
@@ -443,7 +490,8 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const unsigned Lf = COLFAST ? M : N;  // fast (lane) axis length
   const int Ls = COLFAST ? N : M;
-  if (wave >= Ls) return;
+  const int first = wave + FWD_WAVES * CH * static_cast<int>(blockIdx.y);
+  if (first >= Ls) return;
   const float Dmax = static_cast<float>(D);
   const unsigned span = 64u * S;
   const unsigned loff = static_cast<unsigned>(S * ((lane * kmul) & 63));
@@ -467,7 +515,7 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
   };
   // The walk over (line group, block, phase) is flat; the image values of
   // the next phase are loaded before the current phase is accumulated.
-  int s0 = wave, sn = wave;
+  int s0 = first, sn = first;
   unsigned c0 = 0, cn = 0;
   int r = 0, rn = 0;
   set_lines(sn);
@@ -489,7 +537,7 @@ __device__ __forceinline__ void fwd_body(const float* __restrict__ src, float* a
       cn += span;
       if (cn >= Lf) {
         cn = 0;
-        sn += FWD_WAVES * CH;
+        sn += FWD_WAVES * CH * static_cast<int>(gridDim.y);
         if (sn < Ls) set_lines(sn);
       }
     }
@@ -528,7 +576,8 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const unsigned Lf = COLFAST ? M : N;
   const int Ls = COLFAST ? N : M;
-  if (wave >= Ls) return;
+  const int first = wave + FWD_WAVES * CH * static_cast<int>(blockIdx.y);
+  if (first >= Ls) return;
   const float Dmax = static_cast<float>(D);
   constexpr unsigned span = 64u * S;
   const unsigned loff = static_cast<unsigned>(S * ((lane * kmul) & 63));
@@ -587,7 +636,7 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
           b[c][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs[c], o[r], 0, 1));
     }
   };
-  int s0 = wave;
+  int s0 = first;
   unsigned c0 = 0;
   set_lines(s0);
   set_t(s0);
@@ -598,7 +647,7 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
     unsigned cn = c0 + span;
     if (cn >= Lf) {
       cn = 0;
-      sn += FWD_WAVES * CH;
+      sn += FWD_WAVES * CH * static_cast<int>(gridDim.y);
     }
     const bool more = sn < Ls;
     if (more) {
@@ -715,6 +764,10 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   }
 }
 
+// Workgroup (g, part) of a grid (groups, SP) takes every SP-th set of
+// FWD_WAVES * CH lines of group g, from set `part`; with SP > 1 it writes
+// its rows to partial sinogram `part` of `parts` (A x D each), which
+// fwd_parts_kernel adds up.
 template <int V, int CH, bool FR>
 __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_kernel(const float* __restrict__ im,
                                                   const float* __restrict__ imT,
@@ -725,11 +778,49 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_kernel(const float* __rest
                                                   const int* __restrict__ order, int M, int N,
                                                   int D, const float* __restrict__ yres,
                                                   double* __restrict__ res_ws,
+                                                  float* __restrict__ parts, int A,
                                                   const int32_t* pred) {
   if (pred_off(pred)) return;
   extern __shared__ float acc[];  // [V][FWD_WAVES waves][CH chains][L]
-  fwd_group<V, CH, FR>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws,
-                       blockIdx.x, acc);
+  if (gridDim.y > 1)
+    fwd_group<V, CH, FR>(im, imT, parts + static_cast<size_t>(blockIdx.y) * A * D, geom, rwt,
+                         groups, order, M, N, D, nullptr, nullptr, blockIdx.x, acc);
+  else
+    fwd_group<V, CH, FR>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws,
+                         blockIdx.x, acc);
+}
+
+// sino = parts[0] + parts[1] + ... (in part order) for the views of group
+// blockIdx.x, with that group's residual partial as fwd_group writes it.
+__global__ __launch_bounds__(256) void fwd_parts_kernel(const float* __restrict__ parts, int sp,
+                                                        int A, int D, float* __restrict__ sino,
+                                                        const int4* __restrict__ groups,
+                                                        const int* __restrict__ order,
+                                                        const float* __restrict__ yres,
+                                                        double* __restrict__ res_ws,
+                                                        const int32_t* pred) {
+  if (pred_off(pred)) return;
+  __shared__ double red[4];
+  const int gi = blockIdx.x;
+  const int4 grp = groups[gi];
+  const size_t plane = static_cast<size_t>(A) * D;
+  double res = 0.0;
+  for (int v = 0; v < grp.y; ++v) {
+    const size_t row = static_cast<size_t>(order[grp.x + v]) * D;
+    for (int b = threadIdx.x; b < D; b += 256) {
+      float val = parts[row + b];
+      for (int q = 1; q < sp; ++q) val = __fadd_rn(val, parts[q * plane + row + b]);
+      sino[row + b] = val;
+      if (yres) {
+        const double d = static_cast<double>(__fsub_rn(yres[row + b], val));
+        res += d * d;
+      }
+    }
+  }
+  if (res_ws) {
+    res = block_sum_f64(res, red);
+    if (threadIdx.x == 0) res_ws[gi] = res;
+  }
 }
 
 // Dynamic LDS beyond 64 KiB for the one-view, one-chain forward (wide
@@ -1260,16 +1351,17 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
     SCXR_LAUNCHED("xr2d transpose");
   }
   const size_t lds = fwd_lds_bytes(p.vpg, p.D, p.fwd_ch);
+  const dim3 fgrid(p.n_groups, p.fwd_sp);
 #define SCXR_FWD(VV, CC)                                                                      \
   do {                                                                                         \
     if (p.adj_fract)                                                                           \
-      hipLaunchKernelGGL((fwd_kernel<VV, CC, true>), dim3(p.n_groups), dim3(64 * FWD_WAVES), lds, \
-                         s, im, p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, \
-                         yres, res_ws, pred);                                                \
+      hipLaunchKernelGGL((fwd_kernel<VV, CC, true>), fgrid, dim3(64 * FWD_WAVES), lds, s, im,  \
+                         p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,  \
+                         res_ws, p.parts, p.A, pred);                                         \
     else                                                                                       \
-      hipLaunchKernelGGL((fwd_kernel<VV, CC, false>), dim3(p.n_groups), dim3(64 * FWD_WAVES),   \
-                         lds, s, im, p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, \
-                         p.D, yres, res_ws, pred);                                           \
+      hipLaunchKernelGGL((fwd_kernel<VV, CC, false>), fgrid, dim3(64 * FWD_WAVES), lds, s, im, \
+                         p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,  \
+                         res_ws, p.parts, p.A, pred);                                         \
   } while (0)
   if (p.fwd_ch == 2) {
     if (p.vpg == 1)
@@ -1288,6 +1380,11 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
   }
 #undef SCXR_FWD
   SCXR_LAUNCHED("xr2d forward");
+  if (p.fwd_sp > 1) {
+    hipLaunchKernelGGL(fwd_parts_kernel, dim3(p.n_groups), dim3(256), 0, s, p.parts, p.fwd_sp,
+                       p.A, p.D, sino, p.groups, p.order, yres, res_ws, pred);
+    SCXR_LAUNCHED("xr2d forward part sums");
+  }
   return SCXR_OK;
 }
 
@@ -1406,6 +1503,15 @@ int scxr_xr2d_adjoint_epi(const scxr_xr2d_plan* plan, const float* sino, float* 
   e.u = u;
   e.dot_ws = dot_ws;
   return xr2d::adjoint(plan->p, sino, im, pred, as_stream(stream), e);
+}
+
+int scxr_xr2d_set_forward_split(scxr_xr2d_plan* plan, int32_t parts, int32_t* effective) {
+  SCXR_REQUIRE(plan && parts >= 0 && parts <= FWD_SP_MAX, SCXR_ERR_ARG,
+               "forward split must be 0 (automatic) or 1..%d parts", FWD_SP_MAX);
+  auto& p = plan->p;
+  const int rc = xr2d::set_split(p, parts == 0 ? xr2d::auto_split(p) : parts);
+  if (effective) *effective = p.fwd_sp;
+  return rc;
 }
 
 int scxr_xr2d_info(const scxr_xr2d_plan* plan, int32_t* out) {

@@ -21,6 +21,7 @@ constexpr int ADJ_WFAST = 64;  // window slots (one per lane)
 // Forward: FWD_WAVES waves x ch chains (2, or 1 for wide detectors)
 // private LDS copies of (D + 4) bins per owned view.
 constexpr int FWD_WAVES = 8;
+constexpr int FWD_SP_MAX = 8;  // most workgroups one forward group's lines are split over
 constexpr size_t FWD_LDS_MAX = 64 * 1024;
 constexpr int FWD_SMAX = 64;  // largest lane stride of the forward projector
 inline size_t fwd_lds_bytes(int vpg, int D, int ch) {
@@ -47,6 +48,7 @@ struct Plan {
   int vpg = 1;         // views per forward workgroup
   int fwd_ch = 2;      // forward chains per wave
   int n_groups = 0;    // forward workgroups
+  int fwd_sp = 1;      // workgroups per forward group, splitting its lines
   int adj_w = 0;       // adjoint window width (bins)
   int adj_ppt = 1;     // adjoint pixels per thread
   bool adj_fract = false;  // Px >= 1 everywhere: fraction-instruction weights
@@ -61,6 +63,8 @@ struct Plan {
   int* order = nullptr;       // views grouped by class
   float* clo = nullptr;       // adjoint window origin offset per view
   float* work = nullptr;      // transposed image workspace (M * N)
+  float* parts = nullptr;     // fwd_sp_cap partial sinograms (A x D each)
+  int fwd_sp_cap = 1;         // parts the workspace holds (1: none)
   std::vector<float4> host_geom;
 };
 
@@ -151,6 +155,8 @@ int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
 int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* dx, double y0,
            const float* angles);
 void destroy(Plan& p);
+int auto_split(const Plan& p);
+int set_split(Plan& p, int parts);
 int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,
             const float* yres, double* res_ws);
 int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,

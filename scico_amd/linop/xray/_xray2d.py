@@ -128,6 +128,16 @@ class XRayTransform2D(LinearOperator):
         return {"forward_groups": int(out[0]), "adjoint_tiles": int(out[1]),
                 "views_per_group": int(out[2]), "adjoint_window": int(out[3])}
 
+    def set_forward_split(self, parts: int = 0) -> int:
+        """Divide each forward group's image lines over ``parts`` workgroups
+        (1..8; 0 = the automatic choice) and return the count in effect.
+        The parts' partial sinograms are added in a fixed order; a split
+        changes float32 rounding only.  With 1 the forward is bitwise equal
+        to the batched projector's for the same slice."""
+        eff = ctypes.c_int32(0)
+        call("scxr_xr2d_set_forward_split", self.plan, int(parts), ctypes.byref(eff))
+        return int(eff.value)
+
     @property
     def tiled(self) -> bool:
         """Both directions run the tiled kernels (with fused epilogues)."""

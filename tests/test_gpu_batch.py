@@ -3,9 +3,10 @@ ADMM-TV against the single-slice path.
 
 Bar: with one view per forward group (the single-slice choice below 1024
 views) every slice's projection, back projection, ADMM iterate and
-statistics row is bitwise equal to the single-slice result, for every
-item order and persistent grid size; other group sizes stay within
-rel-L2 1e-6.  Graph replays of a batched iteration are bitwise equal to
+statistics row is bitwise equal to the single-slice result with the
+forward line split off (one workgroup per forward group, the batched
+projector's own arithmetic), for every item order and persistent grid
+size; other group sizes stay within rel-L2 1e-6.  Graph replays of a batched iteration are bitwise equal to
 eager iterations.
 """
 
@@ -33,7 +34,9 @@ def _ragged(ns, sizes, amin, amax, seed):
 
 
 def _single(shape, ang):
-    return XRayTransform2D(shape, ang)
+    H = XRayTransform2D(shape, ang)
+    assert H.set_forward_split(1) == 1
+    return H
 
 
 def test_batch_projector_bitwise(gpu):
@@ -46,7 +49,7 @@ def test_batch_projector_bitwise(gpu):
     xb, yb = B.pack_images(ims), B.pack_sinograms(ys)
     refs = []
     for s in range(B.ns):
-        H = XRayTransform2D(shapes[s], angles[s])
+        H = _single(shapes[s], angles[s])
         refs.append((H(ims[s]), H.T(ys[s])))
     for cfg in [(0, 0, 0), (256, 256, 1), (512, 1024, 2), (3, 5, 0)]:
         B.configure(*cfg)
@@ -78,7 +81,7 @@ def _problem(shapes, angles, seed=0):
 
 
 def _single_admm(shape, ang, y, lam, rho, iters, cg_tol):
-    H = XRayTransform2D(shape, ang)
+    H = _single(shape, ang)
     s = ADMM(f=loss.SquaredL2Loss(y=torch.as_tensor(y, device="cuda"), A=H),
              g_list=[lam * functional.L21Norm()],
              C_list=[linop.FiniteDifference(shape, append=0)], rho_list=[rho], maxiter=iters,
@@ -163,7 +166,7 @@ def test_full_batch_256_slices(gpu):
     adj = B.unpack_images(B.back_project(yb))
     ims, sinos = B.unpack_images(xb), B.unpack_sinograms(yb)
     for s in rng.choice(B.ns, 12, replace=False):
-        H = XRayTransform2D(shapes[s], angles[s])
+        H = _single(shapes[s], angles[s])
         assert torch.equal(fwd[s], H(ims[s].contiguous())), s
         assert torch.equal(adj[s], H.T(sinos[s].contiguous())), s
 
@@ -188,7 +191,7 @@ def test_full_batch_256_slices_two_views_per_group(gpu):
     check_adj = set(rng.choice(B.ns, 12, replace=False).tolist())
     worst = 0.0
     for s in range(B.ns):
-        H = XRayTransform2D(shapes[s], angles[s])
+        H = _single(shapes[s], angles[s])
         worst = max(worst, rel_l2(fwd[s].cpu().numpy(), H(ims[s].contiguous()).cpu().numpy()))
         if s in check_adj:
             assert torch.equal(adj[s], H.T(sinos[s].contiguous())), s
@@ -213,7 +216,7 @@ def test_full_batch_256_slices_two_adjoint_groups(gpu):
     check_fwd = set(rng.choice(B.ns, 12, replace=False).tolist())
     worst = 0.0
     for s in range(B.ns):
-        H = XRayTransform2D(shapes[s], angles[s])
+        H = _single(shapes[s], angles[s])
         worst = max(worst, rel_l2(adj[s].cpu().numpy(), H.T(sinos[s].contiguous()).cpu().numpy()))
         if s in check_fwd:
             assert torch.equal(fwd[s], H(ims[s].contiguous())), s

@@ -94,6 +94,56 @@ def test_determinism(gpu):
     assert torch.equal(y1, y2) and torch.equal(b1, b2)
 
 
+@pytest.mark.parametrize("shape,nang,dc", [((256, 256), 90, 363), ((300, 260), 75, 400)])
+def test_forward_split(gpu, shape, nang, dc):
+    """Forward line split: every part count gives the oracle's projection
+    (rel-L2 1e-5) and the unsplit result up to float32 rounding (rel-L2
+    1e-6), deterministically; the residual partials of the split forward
+    sum to the oracle's residual; the automatic choice is 4 parts from
+    256 x 256 pixels up with fewer than two forward groups per CU, else 1."""
+    import ctypes
+
+    from scico_amd import _lib
+
+    ang = phantom.angles(nang)
+    H = XRayTransform2D(shape, ang, det_count=dc)
+    assert H.set_forward_split(0) == 4
+    assert XRayTransform2D((96, 80), ang).set_forward_split(0) == 1
+    assert XRayTransform2D((512, 512), phantom.angles(720), det_count=724).set_forward_split(0) == 1
+    rng = np.random.default_rng(5)
+    x = rng.random(shape).astype(np.float32)
+    y = rng.standard_normal((nang, dc)).astype(np.float32)
+    g = X.xr2d_defaults(shape, ang, det_count=dc)
+    ref = X.project(x, g["x0"], g["dx"], g["y0"], g["ny"], g["angles"])
+    ref_res = float(np.sum((y.astype(np.float64) - ref) ** 2))
+    xd, yd = torch.as_tensor(x, device="cuda"), torch.as_tensor(y, device="cuda")
+    out = (ctypes.c_int32 * 4)()
+    _lib.call("scxr_xr2d_info", H.plan, out)
+    n_fwd = int(out[0])
+    assert n_fwd > 0
+    base = None
+    for parts in (1, 2, 3, 4, 8):
+        assert H.set_forward_split(parts) == parts
+        s1, s2 = H(xd), H(xd)
+        assert torch.equal(s1, s2), parts
+        e = rel_l2(s1.cpu().numpy(), ref)
+        assert e <= 1e-5, (parts, e)
+        if base is None:
+            base = s1.clone()
+        else:
+            assert rel_l2(s1.cpu().numpy(), base.cpu().numpy()) <= 1e-6, parts
+        ws = torch.empty(n_fwd, dtype=torch.float64, device="cuda")
+        sr = torch.empty_like(s1)
+        _lib.call("scxr_xr2d_forward_res", H.plan, xd.data_ptr(), sr.data_ptr(), yd.data_ptr(),
+                  ws.data_ptr(), None, None)
+        assert torch.equal(sr, s1), parts
+        own = float(((yd - sr).double() ** 2).sum())  # float32 differences, as the kernel
+        assert abs(float(ws.sum()) - own) <= 1e-12 * own, parts
+        assert abs(float(ws.sum()) - ref_res) <= 1e-5 * ref_res, parts
+    with pytest.raises(Exception):
+        H.set_forward_split(9)
+
+
 def test_full_size_512_720(gpu):
     """512^2, 720 views, 724 bins: view-subset parity + mass + dot test."""
     N, A, D = 512, 720, 724
