@@ -62,15 +62,19 @@ void view_geometry(int A, const float* angles, const double* x0, const double* d
 // lanes at a time over 32 banks, one word per bank per cycle, so each
 // half-wave and word costs the largest number of its lanes that share a
 // bank; the sum over both halves and both words is returned, averaged over
-// fixed offsets p.  The cost depends only on |d| and k; it is tabulated
-// for |d| in steps of 1/256.
-static int lane_mult_for(double d) {
-  thread_local std::vector<signed char> memo;
+// fixed offsets p.  The cost depends only on |d| and k; the cheapest odd k
+// and its cost are tabulated for |d| in steps of 1/256.
+struct LaneChoice {
+  int k;     // lane multiplier
+  int cost;  // modelled bank cost of a phase (96: two lanes per bank)
+};
+static LaneChoice lane_choice(double d) {
+  thread_local std::vector<int> memo;
   d = std::fabs(d);
   const int key = static_cast<int>(std::lround(d * 256.0));
-  if (key < 0 || key >= 64 * 256) return 1;
+  if (key < 0 || key >= 64 * 256) return {1, 96};
   if (memo.empty()) memo.assign(64 * 256, 0);
-  if (memo[key]) return memo[key];
+  if (memo[key]) return {memo[key] & 255, memo[key] >> 8};
   const double dq = key / 256.0;
   int best_k = 1;
   long best = -1;
@@ -93,8 +97,40 @@ static int lane_mult_for(double d) {
       best_k = k;
     }
   }
-  memo[key] = static_cast<signed char>(best_k);
-  return best_k;
+  memo[key] = best_k | static_cast<int>(best << 8);
+  return {best_k, static_cast<int>(best)};
+}
+
+// Workgroup placement of the forward groups.  The dispatcher hands a
+// launch's workgroups out in index order, one per CU per round while every
+// CU has room (block index i lands on CU slot i mod cus of round
+// i / cus), and a forward group's time varies with its view (1.4x between
+// a view whose lanes fall on distinct LDS banks and one whose bins alias).
+// Each round, the groups still to place, most expensive first, go to the
+// CU slots with the least estimated load so far (ties: higher slots
+// first, since a last, partial round fills only the lowest slots).  Only
+// the order in which groups are processed changes, not their arithmetic.
+static std::vector<int> place_groups(const std::vector<float>& est, int cus) {
+  const int G = static_cast<int>(est.size());
+  std::vector<int> byc(G), block(G);
+  for (int i = 0; i < G; ++i) byc[i] = i;
+  std::stable_sort(byc.begin(), byc.end(), [&](int a, int b) { return est[a] > est[b]; });
+  std::vector<double> load(cus, 0.0);
+  std::vector<int> slot;
+  for (int r = 0, next = 0; next < G; ++r) {
+    const int n = std::min(cus, G - next);
+    slot.resize(n);
+    for (int i = 0; i < n; ++i) slot[i] = i;
+    std::stable_sort(slot.begin(), slot.end(), [&](int a, int b) {
+      return load[a] < load[b] || (load[a] == load[b] && a > b);
+    });
+    for (int j = 0; j < n; ++j) {
+      load[slot[j]] += est[byc[next + j]];
+      block[r * cus + slot[j]] = byc[next + j];
+    }
+    next += n;
+  }
+  return block;
 }
 
 int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, const double* dx,
@@ -125,6 +161,7 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
   }
   h.order.clear();
   h.groups.clear();
+  h.gcost.clear();
   for (int cls = 0; cls < 2 && !h.fwd_generic; ++cls) {
     const std::vector<int>& list = cls ? cols : rows;
     for (size_t s = 0; s < list.size(); s += vpg) {
@@ -160,8 +197,18 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
       // bank model prefers for the group's first view: the phase's set of
       // pixels, hence the disjointness of the footprints, is unchanged.
       const float4 g0 = geom[list[s]];
-      const int kmul = lane_mult_for(S * static_cast<double>(cls ? g0.y : g0.z));
+      const LaneChoice lc = lane_choice(S * static_cast<double>(cls ? g0.y : g0.z));
+      const int kmul = lc.k;
       h.groups.push_back(make_int4(static_cast<int>(h.order.size()), cnt, cls | (kmul << 1), S));
+      // Relative time of the group: phases per line times lines times a
+      // per-phase cost, fitted to 512^2 plans of one repeated view on
+      // MI355X (95-137 us over the angles; the phase time stops falling
+      // below a bank cost of 92, where the VALU work sets it).
+      {
+        const int Lf = cls ? M : N, Ls = cls ? N : M;
+        const double ph = static_cast<double>((Lf + 64 * S - 1) / (64 * S) * S);
+        h.gcost.push_back(static_cast<float>(cnt * Ls * ph * (33.0 + std::max(lc.cost, 92))));
+      }
       for (int k = 0; k < cnt; ++k) h.order.push_back(list[s + k]);
     }
   }
@@ -215,6 +262,7 @@ int build_tables(HostTables& h, int M, int N, int A, int D, const double* x0, co
   }
   if (h.fwd_generic) {
     h.groups.clear();
+    h.gcost.clear();
     h.order.assign(A, 0);
     for (int a = 0; a < A; ++a) h.order[a] = a;
   }
@@ -270,7 +318,19 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   size_t bytes_geom = sizeof(float4) * A, bytes_rw = sizeof(double) * A;
   size_t bytes_groups = sizeof(int4) * groups.size(), bytes_order = sizeof(int) * A;
   size_t bytes_clo = sizeof(float) * A;
-  size_t total = bytes_geom + bytes_rw + bytes_groups + bytes_order + bytes_clo + 256 * 5;
+  // block index -> forward group, balanced over the CUs
+  std::vector<int> gblock;
+  if (!groups.empty()) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    gblock = place_groups(h.gcost, cus);
+  }
+  size_t bytes_gblock = sizeof(int) * gblock.size();
+  size_t total =
+      bytes_geom + bytes_rw + bytes_groups + bytes_order + bytes_clo + bytes_gblock + 256 * 6;
   if (hipMalloc(&p.dev_block, total) != hipSuccess)
     return fail(SCXR_ERR_NOMEM, "hipMalloc of %zu bytes for the plan tables failed", total);
   char* base = static_cast<char*>(p.dev_block);
@@ -284,6 +344,9 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
   p.groups = reinterpret_cast<int4*>(carve(bytes_groups));
   p.order = reinterpret_cast<int*>(carve(bytes_order));
   p.clo = reinterpret_cast<float*>(carve(bytes_clo));
+  p.gblock = reinterpret_cast<int*>(carve(bytes_gblock));
+  if (bytes_gblock)
+    SCXR_HIP(hipMemcpy(p.gblock, gblock.data(), bytes_gblock, hipMemcpyHostToDevice));
   SCXR_HIP(hipMemcpy(p.clo, clo.data(), bytes_clo, hipMemcpyHostToDevice));
   SCXR_HIP(hipMemcpy(p.geom, geom.data(), bytes_geom, hipMemcpyHostToDevice));
   SCXR_HIP(hipMemcpy(p.rw, rw.data(), bytes_rw, hipMemcpyHostToDevice));
@@ -764,8 +827,9 @@ __device__ __forceinline__ void fwd_group(const float* __restrict__ im,
   }
 }
 
-// Workgroup (g, part) of a grid (groups, SP) takes every SP-th set of
-// FWD_WAVES * CH lines of group g, from set `part`; with SP > 1 it writes
+// Workgroup (b, part) of a grid (groups, SP) runs forward group gblock[b]
+// (the CU-balanced placement of the plan), taking every SP-th set of
+// FWD_WAVES * CH lines of it, from set `part`; with SP > 1 it writes
 // its rows to partial sinogram `part` of `parts` (A x D each), which
 // fwd_parts_kernel adds up.
 template <int V, int CH, bool FR>
@@ -779,15 +843,17 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void fwd_kernel(const float* __rest
                                                   int D, const float* __restrict__ yres,
                                                   double* __restrict__ res_ws,
                                                   float* __restrict__ parts, int A,
+                                                  const int* __restrict__ gblock,
                                                   const int32_t* pred) {
   if (pred_off(pred)) return;
   extern __shared__ float acc[];  // [V][FWD_WAVES waves][CH chains][L]
+  const int gi = gblock[blockIdx.x];
   if (gridDim.y > 1)
     fwd_group<V, CH, FR>(im, imT, parts + static_cast<size_t>(blockIdx.y) * A * D, geom, rwt,
-                         groups, order, M, N, D, nullptr, nullptr, blockIdx.x, acc);
+                         groups, order, M, N, D, nullptr, nullptr, gi, acc);
   else
-    fwd_group<V, CH, FR>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws,
-                         blockIdx.x, acc);
+    fwd_group<V, CH, FR>(im, imT, sino, geom, rwt, groups, order, M, N, D, yres, res_ws, gi,
+                         acc);
 }
 
 // sino = parts[0] + parts[1] + ... (in part order) for the views of group
@@ -1357,11 +1423,11 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
     if (p.adj_fract)                                                                           \
       hipLaunchKernelGGL((fwd_kernel<VV, CC, true>), fgrid, dim3(64 * FWD_WAVES), lds, s, im,  \
                          p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,  \
-                         res_ws, p.parts, p.A, pred);                                         \
+                         res_ws, p.parts, p.A, p.gblock, pred);                                         \
     else                                                                                       \
       hipLaunchKernelGGL((fwd_kernel<VV, CC, false>), fgrid, dim3(64 * FWD_WAVES), lds, s, im, \
                          p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,  \
-                         res_ws, p.parts, p.A, pred);                                         \
+                         res_ws, p.parts, p.A, p.gblock, pred);                                         \
   } while (0)
   if (p.fwd_ch == 2) {
     if (p.vpg == 1)

@@ -61,6 +61,7 @@ struct Plan {
   double* rw = nullptr;       // RN64(1 / width) per view
   int4* groups = nullptr;     // forward groups: (first, count, class, 0)
   int* order = nullptr;       // views grouped by class
+  int* gblock = nullptr;      // forward group of each forward block index
   float* clo = nullptr;       // adjoint window origin offset per view
   float* work = nullptr;      // transposed image workspace (M * N)
   float* parts = nullptr;     // fwd_sp_cap partial sinograms (A x D each)
@@ -89,6 +90,7 @@ struct HostTables {
   std::vector<double> rw;
   std::vector<int4> groups;
   std::vector<int> order;
+  std::vector<float> gcost;  // estimated relative time of each forward group
   std::vector<float> clo;
   int adj_w = 0;
   int adj_ppt = 1;
