@@ -274,6 +274,9 @@ class ADMMTVBatch:
         self.r, self.p = torch.empty(n, **f32), torch.empty(n, **f32)
         self.Ap, self.rhs = torch.empty(n, **f32), torch.empty(n, **f32)
         self.sino = torch.empty(H.n_sino, **f32)
+        # A x of the current iterate (x = 0: exactly 0), left by the residual
+        # projection that ends an iteration for the x-step of the next
+        self.ax = torch.zeros(H.n_sino, **f32)
         self.SS = ST_NUM + 2 * (self.maxiter + 1)
         self.state = torch.zeros(ns * self.SS, **f64)
         self.flags = torch.zeros(ns * (self.maxiter + 1), dtype=torch.int32, device=dev.DEVICE)
@@ -290,7 +293,7 @@ class ADMMTVBatch:
     def reset(self):
         """Restart from x = 0 (z = C 0 = 0, u = 0) with empty statistics;
         device fills only, so it may run between graph replays."""
-        for t in (self.x, self.z, self.u, self.zold):
+        for t in (self.x, self.z, self.u, self.zold, self.ax):
             vec.fill(t, 0.0)
         vec.fill(self.counters.view(torch.float32), 0.0)
 
@@ -306,8 +309,7 @@ class ADMMTVBatch:
         F = self.maxiter + 1
         H.back_project(self.y, out=self.rhs, kind=2, rho=self.rho, circular=self.circ, z=self.z,
                        u=self.u)
-        H.project(self.x, out=self.sino)
-        H.back_project(self.sino, out=self.Ap, kind=1, rho=self.rho, circular=self.circ, p=self.x)
+        H.back_project(self.ax, out=self.Ap, kind=1, rho=self.rho, circular=self.circ, p=self.x)
         call("scxr_batch_cg_start", H.plan, P(self.rhs), P(self.Ap), P(self.r), P(self.p),
              P(self.state), P(self.flags), self.maxiter, self.tol, self.atol, P(self.ws), s)
         for k in range(self.maxiter):
@@ -321,7 +323,7 @@ class ADMMTVBatch:
              self.thr, self.circ, int(self.iso), P(self.ws_prml), P(self.ws_g), s)
         call("scxr_batch_admm_dual", H.plan, P(self.z), P(self.zold), self.rho, self.circ,
              P(self.ws_dual), s)
-        H.project(self.x, out=self.sino, y=self.y, res_ws=self.ws_res)
+        H.project(self.x, out=self.ax, y=self.y, res_ws=self.ws_res)
         call("scxr_batch_admm_itstat", H.plan, P(self.ws_res), P(self.ws_g), P(self.ws_prml),
              P(self.ws_dual), P(self.state), self.maxiter, self.lam, self.rho, P(self.hist),
              P(self.counters), self.hist_cap, s)

@@ -234,6 +234,10 @@ class _FusedTV:
         self.Ap = torch.empty_like(self.r)
         self.rhs = torch.empty_like(self.r)
         self.sino = torch.empty(self.A.output_shape, dtype=torch.float32, device=dev.DEVICE)
+        # A x of the current iterate: written by the residual projection that
+        # ends an iteration and read by the x-step operator of the next (x
+        # does not change in between), so an iteration projects x once
+        self.ax = torch.empty_like(self.sino)
         self.pap = torch.empty(self.n_adj, **f64)
         self.ws_prml = torch.empty(self.G, **f64)
         self.ws_g = torch.empty(self.G, **f64)
@@ -256,6 +260,15 @@ class _FusedTV:
         self.u = a.u_list[0].contiguous().clone()
         self.zold = a.z_list_old[0].contiguous().clone()
         self._publish()
+        self._project_x()
+
+    def _project_x(self):
+        """ax = A x for the iterate just set (device work only)."""
+        if self.epi:
+            call("scxr_xr2d_forward", self.A.plan, self.x.data_ptr(), self.ax.data_ptr(), None,
+                 dev.stream())
+        else:
+            self._fwd(self.x, out=self.ax)
 
     def _publish(self):
         a = self.admm
@@ -275,6 +288,7 @@ class _FusedTV:
         self.C._eval(self.x, out=self.z)
         vec.copy(self.z, out=self.zold)
         vec.fill(self.u, 0.0)
+        self._project_x()
         vec.fill(self.counter.view(torch.float32), 0.0)
         self.n_steps = 0
         self.has_stats = False
@@ -302,7 +316,9 @@ class _FusedTV:
                                     "rel_res": self.hist[self.n_steps - 1, 4]}
 
     def enqueue(self):
-        """Enqueue one full ADMM iteration (no host synchronisation; capturable)."""
+        """Enqueue one full ADMM iteration (no host synchronisation; capturable).
+        It starts from ``ax`` = A x, left by the previous iteration (or by
+        ``adopt_state`` / ``reset``), and leaves A x of the new iterate there."""
         if not self.epi:
             self._enqueue_nd()
             return
@@ -312,9 +328,8 @@ class _FusedTV:
         # x-step: rhs = A^T y + rho C^T (z - u)
         call("scxr_xr2d_adjoint_epi", plan, P(y), P(self.rhs), 2, self.rho, self.circ, None,
              P(self.z), P(self.u), None, None, s)
-        # A_lhs x0 = rho C^T C x + A^T A x
-        call("scxr_xr2d_forward", plan, P(self.x), P(self.sino), None, s)
-        call("scxr_xr2d_adjoint_epi", plan, P(self.sino), P(self.Ap), 1, self.rho, self.circ,
+        # A_lhs x0 = rho C^T C x + A^T A x, with A x from the last iteration
+        call("scxr_xr2d_adjoint_epi", plan, P(self.ax), P(self.Ap), 1, self.rho, self.circ,
              P(self.x), None, None, None, None, s)
         cg = self.cg
         cg.start(self.rhs, self.Ap, self.r, self.p, self.tol, self.atol)
@@ -329,7 +344,7 @@ class _FusedTV:
              self.thr, self.circ, int(self.iso), P(self.ws_prml), P(self.ws_g), s)
         call("scxr_admm_dual2d", self.M, self.N, P(self.z), P(self.zold), self.rho, self.circ,
              P(self.ws_dual), s)
-        call("scxr_xr2d_forward_res", plan, P(self.x), P(self.sino), P(y), P(self.ws_res), None,
+        call("scxr_xr2d_forward_res", plan, P(self.x), P(self.ax), P(y), P(self.ws_res), None,
              s)
         self._itstat()
 
@@ -348,8 +363,7 @@ class _FusedTV:
         self._bwd(y, out=self.rhs)
         call("scxr_admm_rhs_nd", nd, sh, P(self.rhs), P(self.z), P(self.u), self.rho, self.circ,
              P(self.rhs), s)
-        self._fwd(self.x, out=self.sino)
-        self._bwd(self.sino, out=self.Ap)
+        self._bwd(self.ax, out=self.Ap)  # A x from the last iteration
         call("scxr_admm_lhs_nd", nd, sh, P(self.x), P(self.Ap), self.rho, self.circ, P(self.Ap),
              None, None, s)
         cg = self.cg
@@ -365,6 +379,6 @@ class _FusedTV:
              self.circ, int(self.iso), P(self.ws_prml), P(self.ws_g), s)
         call("scxr_admm_dual_nd", nd, sh, P(self.z), P(self.zold), self.rho, self.circ,
              P(self.ws_dual), s)
-        self._fwd(self.x, out=self.sino)
-        vec.reduce(vec.RED_SQDIST, self.sino, y, out=self.ws_res[0])
+        self._fwd(self.x, out=self.ax)
+        vec.reduce(vec.RED_SQDIST, self.ax, y, out=self.ws_res[0])
         self._itstat()
