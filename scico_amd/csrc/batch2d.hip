@@ -41,6 +41,7 @@ struct Batch {
   std::vector<int64_t> ioff, soff;
   std::vector<int> voff, goff, toff, ngroups, ntiles;
   std::vector<int4> groups;
+  std::vector<float> gcost;  // estimated relative time of each forward group
   std::vector<int2> fitems_nat, aitems_nat;
   std::vector<int4> titems;
   int2* fitems = nullptr;  // device item arrays (reordered by configure)
@@ -48,7 +49,8 @@ struct Batch {
 };
 
 // Item orders: 0 slice-major, 1 round robin over slices, 2 decreasing cost
-// (forward group ~ pixels x views, adjoint tile ~ views).
+// (forward group: the plan's estimate, phases x lines x modelled LDS bank
+// cost of its views; adjoint tile ~ views).
 static std::vector<int2> order_items(const Batch& b, const std::vector<int2>& nat, bool fwd,
                                      int mode) {
   if (mode == 0) return nat;
@@ -75,8 +77,7 @@ static std::vector<int2> order_items(const Batch& b, const std::vector<int2>& na
     const int4 sh = b.shape[w.x];
     double cost;
     if (fwd) {
-      const int4 g = b.groups[b.goff[w.x] + w.y];
-      cost = static_cast<double>(sh.x) * sh.y * g.y;
+      cost = b.gcost[b.goff[w.x] + w.y];
     } else {
       cost = static_cast<double>(sh.z);
     }
@@ -172,6 +173,7 @@ int create(Batch& b, int ns, const int32_t* dims, const float* angles, const dou
     order.insert(order.end(), h.order.begin(), h.order.end());
     clo.insert(clo.end(), h.clo.begin(), h.clo.end());
     b.groups.insert(b.groups.end(), h.groups.begin(), h.groups.end());
+    b.gcost.insert(b.gcost.end(), h.gcost.begin(), h.gcost.end());
     const int Gs = static_cast<int>(solver::elem_grid(static_cast<int64_t>(M) * N));
     MN.push_back(make_int2(M, N));
     G.push_back(Gs);
