@@ -357,10 +357,7 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
       return fail(SCXR_ERR_NOMEM, "hipMalloc of the transpose workspace failed");
   }
   p.fwd_sp = 1;
-  if (set_split(p, auto_split(p)) != SCXR_OK) {
-    (void)hipGetLastError();
-    p.fwd_sp = 1;  // no workspace: one workgroup per group
-  }
+  (void)set_split(p, auto_split(p));  // no workspace: one workgroup per group
   return SCXR_OK;
 }
 
@@ -394,8 +391,10 @@ int set_split(Plan& p, int parts) {
   if (p.fwd_generic) parts = 1;
   if (parts > p.fwd_sp_cap) {
     float* w = nullptr;
-    if (hipMalloc(&w, sizeof(float) * parts * static_cast<size_t>(p.A) * p.D) != hipSuccess)
+    if (hipMalloc(&w, sizeof(float) * parts * static_cast<size_t>(p.A) * p.D) != hipSuccess) {
+      (void)hipGetLastError();  // the plan keeps its current split; later launches stay clean
       return fail(SCXR_ERR_NOMEM, "hipMalloc of the %d-part forward workspace failed", parts);
+    }
     if (p.parts) (void)hipFree(p.parts);
     p.parts = w;
     p.fwd_sp_cap = parts;
