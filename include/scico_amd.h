@@ -83,6 +83,13 @@ int scxr_xr2d_view_table(const scxr_xr2d_plan* plan, float* out);
 /* sino = A im  (_xray2d.py:223-265).  Deterministic; no global atomics. */
 int scxr_xr2d_forward(const scxr_xr2d_plan* plan, const float* im, float* sino,
                       const int32_t* pred, void* stream);
+/* scxr_xr2d_forward with the image's transpose imT (N x M) supplied by the
+ * caller (e.g. by scxr_cg_iter_t) instead of computed by a transpose pass;
+ * the same result bit for bit.  imT is read only when the plan has
+ * column-fast views.  No counterpart in the reference (the CG loop's
+ * fusion of the transpose into the direction update). */
+int scxr_xr2d_forward_t(const scxr_xr2d_plan* plan, const float* im, const float* imT,
+                        float* sino, const int32_t* pred, void* stream);
 
 /* im = A^T sino  (_xray2d.py:267-306), gather form, exact adjoint of the
  * forward's weights. */
@@ -236,6 +243,14 @@ int scxr_cg_start(int64_t n, const float* b, const float* Ax, float* r, float* p
 int scxr_cg_iter(int64_t n, int32_t k, int32_t maxiter, float* x, float* r, float* p,
                  const float* Ap, double* state, int32_t* flags, const double* pap_parts,
                  int32_t pap_n, double* ws, void* stream);
+/* scxr_cg_iter for an M x N image whose next direction p also feeds a 2D
+ * projector: the update p = r + beta p additionally writes pT (N x M, the
+ * transpose of p) for scxr_xr2d_forward_t, saving the projector's own
+ * transpose pass.  Same results as scxr_cg_iter bit for bit (no
+ * counterpart in the reference: a fusion of two device passes). */
+int scxr_cg_iter_t(int32_t M, int32_t N, int32_t k, int32_t maxiter, float* x, float* r,
+                   float* p, float* pT, const float* Ap, double* state, int32_t* flags,
+                   const double* pap_parts, int32_t pap_n, double* ws, void* stream);
 /* Workgroups (= partial count) of the elementwise passes over n elements. */
 int32_t scxr_elem_blocks(int64_t n);
 

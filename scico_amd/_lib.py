@@ -56,6 +56,7 @@ SIGNATURES = [
     ("scxr_xr2d_forward", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_adjoint", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_weights", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("scxr_xr2d_forward_t", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("scxr_xr2d_info", c_int32, [c_void_p, c_int32_p]),
     ("scxr_xr2d_set_forward_split", c_int32, [c_void_p, c_int32, c_int32_p]),
     ("scxr_xr2d_forward_res", c_int32,
@@ -115,6 +116,9 @@ SIGNATURES = [
     ("scxr_cg_iter", c_int32,
      [c_int64, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_int32, c_void_p, c_void_p]),
+    ("scxr_cg_iter_t", c_int32,
+     [c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
     ("scxr_elem_blocks", c_int32, [c_int64]),
     # ADMM
     ("scxr_admm_zu2d", c_int32,

@@ -284,6 +284,47 @@ __global__ __launch_bounds__(256) void cg_p_kernel(int64_t n, int k, int maxiter
   cg_p_body(n, k, maxiter, r, p, state, flags, ws_rr, rr_n, blockIdx.x, gridDim.x);
 }
 
+// cg_p_body over a 32 x 32 tile of an M x N image (grid: ceil(N/32) x
+// ceil(M/32), 256 threads), also writing the transposed direction pT
+// (N x M) that the 2D projector's column-fast views stream: p is updated
+// with the same operations, so it is bitwise cg_p_kernel's.
+__global__ __launch_bounds__(256) void cg_p_t_kernel(int M, int N, int k, int maxiter,
+                                                     const float* __restrict__ r,
+                                                     float* __restrict__ p,
+                                                     float* __restrict__ pT,
+                                                     double* __restrict__ state,
+                                                     int32_t* __restrict__ flags,
+                                                     const double* __restrict__ ws_rr, int rr_n) {
+  if (flags[k] == 0) return;
+  __shared__ double sh[4];
+  __shared__ float tile[32][33];
+  const double rr = all_sum_f64(ws_rr, rr_n, sh);
+  const float num_new = static_cast<float>(rr);
+  const float beta = num_new / static_cast<float>(state[ST_NUM + k]);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+  for (int q = ty; q < 32; q += 8) {
+    const int i = i0 + q, j = j0 + tx;
+    if (i < M && j < N) {
+      const int64_t e = static_cast<int64_t>(i) * N + j;
+      const float v = __fadd_rn(r[e], __fmul_rn(beta, p[e]));
+      p[e] = v;
+      tile[q][tx] = v;
+    }
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int j = j0 + q, i = i0 + tx;
+    if (i < M && j < N) pT[static_cast<int64_t>(j) * M + i] = tile[tx][q];
+  }
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    state[ST_NUM + k + 1] = rr;
+    state[ST_NITER] = k + 1;
+    state[ST_RELRES] = sqrt(rr) / state[ST_BN];
+    flags[k + 1] = (k + 1 < maxiter && rr > state[ST_TOL2]) ? 1 : 0;
+  }
+}
+
 // ---------------------------------------------------------------- ADMM --
 
 // One fused pass of the ADMM z/u update for C = FiniteDifference (2D,
@@ -1273,6 +1314,24 @@ int scxr_cg_iter(int64_t n, int32_t k, int32_t maxiter, float* x, float* r, floa
   hipLaunchKernelGGL(solver::cg_p_kernel, dim3(g), dim3(256), 0, as_stream(stream), n, k, maxiter,
                      r, p, state, flags, ws, static_cast<int>(g));
   SCXR_LAUNCHED("cg p");
+  return SCXR_OK;
+}
+
+int scxr_cg_iter_t(int32_t M, int32_t N, int32_t k, int32_t maxiter, float* x, float* r,
+                   float* p, float* pT, const float* Ap, double* state, int32_t* flags,
+                   const double* pap_parts, int32_t pap_n, double* ws, void* stream) {
+  SCXR_REQUIRE(M > 0 && N > 0 && x && r && p && pT && Ap && state && flags && pap_parts && ws,
+               SCXR_ERR_ARG, "bad cg_iter_t arguments");
+  SCXR_REQUIRE(k >= 0 && k < maxiter && pap_n >= 1, SCXR_ERR_ARG, "bad cg_iter_t index");
+  const int64_t n = static_cast<int64_t>(M) * N;
+  const unsigned g = solver::elem_grid(n);
+  hipLaunchKernelGGL(solver::cg_xr_kernel, dim3(g), dim3(256), 0, as_stream(stream), n, k,
+                     maxiter, x, r, p, Ap, state, flags, pap_parts, pap_n, ws);
+  SCXR_LAUNCHED("cg xr");
+  hipLaunchKernelGGL(solver::cg_p_t_kernel, dim3(ceil_div(N, 32), ceil_div(M, 32)), dim3(256), 0,
+                     as_stream(stream), M, N, k, maxiter, r, p, pT, state, flags, ws,
+                     static_cast<int>(g));
+  SCXR_LAUNCHED("cg p (with transposed copy)");
   return SCXR_OK;
 }
 

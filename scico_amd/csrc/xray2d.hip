@@ -1400,7 +1400,7 @@ int batch_adjoint(const BatchTabs& t, int grid, const float* sino, float* im,
 }
 
 int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,
-            const float* yres, double* res_ws) {
+            const float* yres, double* res_ws, const float* imT) {
   if (p.fwd_generic) {
     SCXR_REQUIRE(!res_ws, SCXR_ERR_UNSUPPORTED,
                  "residual partials are not available for this geometry (general-geometry "
@@ -1410,7 +1410,10 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
     SCXR_LAUNCHED("xr2d general forward");
     return SCXR_OK;
   }
-  if (p.any_colfast) {
+  // column-fast views stream the transposed image: the caller's, or the
+  // plan's workspace filled here
+  const float* srcT = imT ? imT : p.work;
+  if (p.any_colfast && !imT) {
     dim3 g(ceil_div(p.N, 32), ceil_div(p.M, 32));
     hipLaunchKernelGGL(transpose_kernel, g, dim3(256), 0, s, im, p.work, p.M, p.N, pred);
     SCXR_LAUNCHED("xr2d transpose");
@@ -1421,12 +1424,12 @@ int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hi
   do {                                                                                         \
     if (p.adj_fract)                                                                           \
       hipLaunchKernelGGL((fwd_kernel<VV, CC, true>), fgrid, dim3(64 * FWD_WAVES), lds, s, im,  \
-                         p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,  \
-                         res_ws, p.parts, p.A, p.gblock, pred);                                         \
+                         srcT, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,    \
+                         res_ws, p.parts, p.A, p.gblock, pred);                               \
     else                                                                                       \
       hipLaunchKernelGGL((fwd_kernel<VV, CC, false>), fgrid, dim3(64 * FWD_WAVES), lds, s, im, \
-                         p.work, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,  \
-                         res_ws, p.parts, p.A, p.gblock, pred);                                         \
+                         srcT, sino, p.geom, p.rw, p.groups, p.order, p.M, p.N, p.D, yres,    \
+                         res_ws, p.parts, p.A, p.gblock, pred);                               \
   } while (0)
   if (p.fwd_ch == 2) {
     if (p.vpg == 1)
@@ -1535,6 +1538,12 @@ int scxr_xr2d_forward(const scxr_xr2d_plan* plan, const float* im, float* sino,
                       const int32_t* pred, void* stream) {
   SCXR_REQUIRE(plan && im && sino, SCXR_ERR_ARG, "null argument to xr2d_forward");
   return xr2d::forward(plan->p, im, sino, pred, as_stream(stream), nullptr, nullptr);
+}
+
+int scxr_xr2d_forward_t(const scxr_xr2d_plan* plan, const float* im, const float* imT,
+                        float* sino, const int32_t* pred, void* stream) {
+  SCXR_REQUIRE(plan && im && imT && sino, SCXR_ERR_ARG, "null argument to xr2d_forward_t");
+  return xr2d::forward(plan->p, im, sino, pred, as_stream(stream), nullptr, nullptr, imT);
 }
 
 int scxr_xr2d_forward_res(const scxr_xr2d_plan* plan, const float* im, float* sino,

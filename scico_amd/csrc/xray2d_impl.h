@@ -159,8 +159,10 @@ int create(Plan& p, int M, int N, int A, int D, const double* x0, const double* 
 void destroy(Plan& p);
 int auto_split(const Plan& p);
 int set_split(Plan& p, int parts);
+// imT: the image transposed (N x M) by the caller, or null to transpose it
+// into the plan's workspace first.
 int forward(const Plan& p, const float* im, float* sino, const int32_t* pred, hipStream_t s,
-            const float* yres, double* res_ws);
+            const float* yres, double* res_ws, const float* imT = nullptr);
 int adjoint(const Plan& p, const float* sino, float* im, const int32_t* pred, hipStream_t s,
             const AdjEpi& epi);
 

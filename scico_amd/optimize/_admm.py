@@ -231,6 +231,10 @@ class _FusedTV:
         self.cg = CGWork(n, self.cg_maxiter)
         self.r = torch.empty(shape, dtype=torch.float32, device=dev.DEVICE)
         self.p = torch.empty_like(self.r)
+        # p transposed, written by the CG direction update for the 2D
+        # projector's column-fast views (saves its transpose pass)
+        self.pT = torch.empty(tuple(reversed(shape)), dtype=torch.float32,
+                              device=dev.DEVICE) if self.epi else None
         self.Ap = torch.empty_like(self.r)
         self.rhs = torch.empty_like(self.r)
         self.sino = torch.empty(self.A.output_shape, dtype=torch.float32, device=dev.DEVICE)
@@ -333,12 +337,19 @@ class _FusedTV:
              P(self.x), None, None, None, None, s)
         cg = self.cg
         cg.start(self.rhs, self.Ap, self.r, self.p, self.tol, self.atol)
+        last = self.cg_maxiter - 1
         for k in range(self.cg_maxiter):
             flag = cg.flags[k:].data_ptr()
-            call("scxr_xr2d_forward", plan, P(self.p), P(self.sino), flag, s)
+            if k == 0:
+                call("scxr_xr2d_forward", plan, P(self.p), P(self.sino), flag, s)
+            else:  # p transposed by the previous step's direction update
+                call("scxr_xr2d_forward_t", plan, P(self.p), P(self.pT), P(self.sino), flag, s)
             call("scxr_xr2d_adjoint_epi", plan, P(self.sino), P(self.Ap), 1, self.rho,
                  self.circ, P(self.p), None, None, P(self.pap), flag, s)
-            cg.iterate(k, self.x, self.r, self.p, self.Ap, self.pap, self.n_adj)
+            if k < last:
+                cg.iterate_t(k, self.x, self.r, self.p, self.pT, self.Ap, self.pap, self.n_adj)
+            else:
+                cg.iterate(k, self.x, self.r, self.p, self.Ap, self.pap, self.n_adj)
         # z / u update and iteration statistics
         call("scxr_admm_zu2d", self.M, self.N, P(self.x), P(self.z), P(self.u), P(self.zold),
              self.thr, self.circ, int(self.iso), P(self.ws_prml), P(self.ws_g), s)

@@ -41,6 +41,14 @@ class CGWork:
              Ap.data_ptr(), self.state.data_ptr(), self.flags.data_ptr(), pap_parts.data_ptr(),
              pap_n, self.ws.data_ptr(), dev.stream())
 
+    def iterate_t(self, k, x, r, p, pT, Ap, pap_parts, pap_n):
+        """``iterate`` for an M x N image ``p`` that also writes its transpose
+        ``pT`` (N x M) for ``scxr_xr2d_forward_t``; bitwise the same x, r, p."""
+        M, N = p.shape
+        call("scxr_cg_iter_t", M, N, k, self.maxiter, x.data_ptr(), r.data_ptr(), p.data_ptr(),
+             pT.data_ptr(), Ap.data_ptr(), self.state.data_ptr(), self.flags.data_ptr(),
+             pap_parts.data_ptr(), pap_n, self.ws.data_ptr(), dev.stream())
+
     def info(self):
         return {"num_iter": self.state[ST_NITER], "rel_res": self.state[ST_RELRES]}
 

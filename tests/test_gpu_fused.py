@@ -151,3 +151,43 @@ def test_one_cg_step(setup):
     assert int(w.info()["num_iter"].item()) == 1
     assert rel_l2(x.cpu().numpy(), xr) <= 1e-6
     assert abs(float(w.info()["rel_res"].item()) - inf["rel_res"]) <= 1e-5 * inf["rel_res"]
+
+
+def test_cg_iter_t_and_forward_t(gpu):
+    """The CG direction update that also writes p transposed gives x, r, p,
+    the CG state and flags bitwise equal to scxr_cg_iter's, and pT equal to
+    p.T; the forward fed that transpose equals the plain forward bitwise
+    (edge tiles: 70 x 45 image; views of both streaming classes)."""
+    from scico_amd.solver import CGWork
+
+    M_, N_ = 70, 45
+    rng = np.random.default_rng(17)
+    b, Ax, Ap = (dev(rng.standard_normal((M_, N_)).astype(np.float32)) for _ in range(3))
+    out = {}
+    for fused in (False, True):
+        cg = CGWork(M_ * N_, 3)
+        x = torch.zeros((M_, N_), device="cuda")
+        r, p = torch.empty_like(x), torch.empty_like(x)
+        pT = torch.full((N_, M_), -7.0, device="cuda")
+        cg.start(b, Ax, r, p, 0.0, 0.0)
+        pap = torch.tensor([2.5 * float((b.double() ** 2).sum())], dtype=torch.float64,
+                           device="cuda")
+        for k in range(2):
+            if fused:
+                cg.iterate_t(k, x, r, p, pT, Ap, pap, 1)
+            else:
+                cg.iterate(k, x, r, p, Ap, pap, 1)
+        out[fused] = (x.clone(), r.clone(), p.clone(), cg.state.clone(), cg.flags.clone(), pT)
+    for a, b2 in zip(out[False][:5], out[True][:5]):
+        assert torch.equal(a, b2)
+    assert torch.equal(out[True][5], out[True][2].t().contiguous())
+    ang = np.linspace(0, np.pi, 30, endpoint=False).astype(np.float32)
+    H = XRayTransform2D((M_, N_), ang)
+    im = dev(rng.random((M_, N_)).astype(np.float32))
+    imT = im.t().contiguous()
+    s1 = torch.empty(H.output_shape, device="cuda")
+    s2 = torch.full(H.output_shape, -1.0, device="cuda")
+    _lib.call("scxr_xr2d_forward", H.plan, im.data_ptr(), s1.data_ptr(), None, None)
+    _lib.call("scxr_xr2d_forward_t", H.plan, im.data_ptr(), imT.data_ptr(), s2.data_ptr(), None,
+              None)
+    assert torch.equal(s1, s2)
