@@ -470,6 +470,11 @@ int scxr_batch_cg_iter(const scxr_xr2d_batch* plan, int32_t k, int32_t maxiter, 
 int scxr_batch_admm_zu(const scxr_xr2d_batch* plan, const float* x, float* z, float* u,
                        float* zold, float thr, int32_t circular, int32_t isotropic,
                        double* ws_prml, double* ws_g, void* stream);
+/* out = aty + rho C^T (z - u) per slice from a kept aty = A^T y: the same
+ * values as scxr_xr2d_batch_adjoint's right-hand-side epilogue (kind 2,
+ * _admmaux.py:245-256) without back projecting y again. */
+int scxr_batch_admm_rhs(const scxr_xr2d_batch* plan, const float* aty, const float* z,
+                        const float* u, float rho, int32_t circular, float* out, void* stream);
 int scxr_batch_admm_dual(const scxr_xr2d_batch* plan, const float* z, const float* zold,
                          float rho, int32_t circular, double* ws, void* stream);
 int scxr_batch_admm_itstat(const scxr_xr2d_batch* plan, const double* res, const double* g,

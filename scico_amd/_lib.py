@@ -86,6 +86,8 @@ SIGNATURES = [
     ("scxr_batch_admm_zu", c_int32,
      [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32, c_void_p,
       c_void_p, c_void_p]),
+    ("scxr_batch_admm_rhs", c_int32,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
     ("scxr_batch_admm_dual", c_int32,
      [c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
     ("scxr_batch_admm_itstat", c_int32,

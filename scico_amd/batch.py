@@ -277,6 +277,11 @@ class ADMMTVBatch:
         # A x of the current iterate (x = 0: exactly 0), left by the residual
         # projection that ends an iteration for the x-step of the next
         self.ax = torch.zeros(H.n_sino, **f32)
+        # A^T y for the x-step right-hand side (y is fixed), back projected
+        # again only when the plan's configuration changes (the adjoint's view
+        # groups decide its rounding)
+        self.aty = torch.empty(n, **f32)
+        self._aty_key = None
         self.SS = ST_NUM + 2 * (self.maxiter + 1)
         self.state = torch.zeros(ns * self.SS, **f64)
         self.flags = torch.zeros(ns * (self.maxiter + 1), dtype=torch.int32, device=dev.DEVICE)
@@ -307,8 +312,11 @@ class ADMMTVBatch:
             raise RuntimeError("the batch plan was re-created (autotune) after this solver was "
                                "built; build a new ADMMTVBatch")
         F = self.maxiter + 1
-        H.back_project(self.y, out=self.rhs, kind=2, rho=self.rho, circular=self.circ, z=self.z,
-                       u=self.u)
+        if self._aty_key != H.config:
+            H.back_project(self.y, out=self.aty)
+            self._aty_key = H.config
+        call("scxr_batch_admm_rhs", H.plan, P(self.aty), P(self.z), P(self.u), self.rho,
+             self.circ, P(self.rhs), s)
         H.back_project(self.ax, out=self.Ap, kind=1, rho=self.rho, circular=self.circ, p=self.x)
         call("scxr_batch_cg_start", H.plan, P(self.rhs), P(self.Ap), P(self.r), P(self.p),
              P(self.state), P(self.flags), self.maxiter, self.tol, self.atol, P(self.ws), s)

@@ -432,6 +432,12 @@ int scxr_batch_admm_zu(const scxr_xr2d_batch* plan, const float* x, float* z, fl
                                ws_g, as_stream(stream));
 }
 
+int scxr_batch_admm_rhs(const scxr_xr2d_batch* plan, const float* aty, const float* z,
+                        const float* u, float rho, int32_t circular, float* out, void* stream) {
+  SCXR_REQUIRE(plan && aty && z && u && out, SCXR_ERR_ARG, "bad batch_admm_rhs arguments");
+  return solver::batch_admm_rhs(plan->b.st, aty, z, u, rho, circular, out, as_stream(stream));
+}
+
 int scxr_batch_admm_dual(const scxr_xr2d_batch* plan, const float* z, const float* zold,
                          float rho, int32_t circular, double* ws, void* stream) {
   SCXR_REQUIRE(plan && z && zold && ws, SCXR_ERR_ARG, "bad batch_admm_dual arguments");

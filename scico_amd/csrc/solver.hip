@@ -1153,6 +1153,22 @@ __global__ __launch_bounds__(256) void admm_dual_b(SegTabs t, const float* __res
                    q.vg);
 }
 
+// out = aty + rho C^T (z - u) per slice: the operations of the 2D adjoint's
+// right-hand-side epilogue (_admmaux.py:245-256), from a kept A^T y.
+__global__ __launch_bounds__(256) void admm_rhs_b(SegTabs t, const float* __restrict__ aty,
+                                                  const float* __restrict__ z,
+                                                  const float* __restrict__ u, float rho,
+                                                  int circ, float* __restrict__ out) {
+  const SegBlock q = seg_block(t);
+  const float* zs = z + 2 * q.o;
+  const float* us = u + 2 * q.o;
+  for (int64_t e = q.vb * 256ll + threadIdx.x; e < q.n; e += 256ll * q.vg) {
+    const int i = static_cast<int>(e / q.N), j = static_cast<int>(e - static_cast<int64_t>(i) * q.N);
+    out[q.o + e] = __fadd_rn(aty[q.o + e],
+                             __fmul_rn(rho, fd::adj2d_diff(zs, us, q.M, q.N, i, j, circ != 0)));
+  }
+}
+
 __global__ __launch_bounds__(256) void admm_itstat_b(
     SegTabs t, const double* __restrict__ res, const double* __restrict__ g,
     const double* __restrict__ prml, const double* __restrict__ dual,
@@ -1194,6 +1210,13 @@ int batch_admm_zu(const SegTabs& t, const float* x, float* z, float* u, float* z
   hipLaunchKernelGGL(admm_zu_b, dim3(t.nblk), dim3(256), 0, s, t, x, z, u, zold, thr, circ, iso,
                      ws_prml, ws_g);
   SCXR_LAUNCHED("batch admm zu");
+  return SCXR_OK;
+}
+
+int batch_admm_rhs(const SegTabs& t, const float* aty, const float* z, const float* u, float rho,
+                   int circ, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(admm_rhs_b, dim3(t.nblk), dim3(256), 0, s, t, aty, z, u, rho, circ, out);
+  SCXR_LAUNCHED("batch admm rhs");
   return SCXR_OK;
 }
 

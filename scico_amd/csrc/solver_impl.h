@@ -44,6 +44,8 @@ int batch_cg_iter(const SegTabs& t, int k, int maxiter, float* x, float* r, floa
                   double* ws, hipStream_t s);
 int batch_admm_zu(const SegTabs& t, const float* x, float* z, float* u, float* zold, float thr,
                   int circ, int iso, double* ws_prml, double* ws_g, hipStream_t s);
+int batch_admm_rhs(const SegTabs& t, const float* aty, const float* z, const float* u, float rho,
+                   int circ, float* out, hipStream_t s);
 int batch_admm_dual(const SegTabs& t, const float* z, const float* zold, float rho, int circ,
                     double* ws, hipStream_t s);
 int batch_admm_itstat(const SegTabs& t, const double* res, const double* g, const double* prml,

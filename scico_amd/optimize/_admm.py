@@ -242,6 +242,10 @@ class _FusedTV:
         # ends an iteration and read by the x-step operator of the next (x
         # does not change in between), so an iteration projects x once
         self.ax = torch.empty_like(self.sino)
+        # A^T y: y is fixed during a solve, so the x-step right-hand side
+        # A^T y + rho C^T (z - u) takes it from here instead of back
+        # projecting y every iteration (refreshed by adopt_state / reset)
+        self.aty = torch.empty_like(self.r)
         self.pap = torch.empty(self.n_adj, **f64)
         self.ws_prml = torch.empty(self.G, **f64)
         self.ws_g = torch.empty(self.G, **f64)
@@ -267,12 +271,16 @@ class _FusedTV:
         self._project_x()
 
     def _project_x(self):
-        """ax = A x for the iterate just set (device work only)."""
+        """ax = A x for the iterate just set and aty = A^T y (device work only)."""
+        y = self.admm.f.y
         if self.epi:
             call("scxr_xr2d_forward", self.A.plan, self.x.data_ptr(), self.ax.data_ptr(), None,
                  dev.stream())
+            call("scxr_xr2d_adjoint", self.A.plan, y.data_ptr(), self.aty.data_ptr(), None,
+                 dev.stream())
         else:
             self._fwd(self.x, out=self.ax)
+            self._bwd(y, out=self.aty)
 
     def _publish(self):
         a = self.admm
@@ -322,16 +330,19 @@ class _FusedTV:
     def enqueue(self):
         """Enqueue one full ADMM iteration (no host synchronisation; capturable).
         It starts from ``ax`` = A x, left by the previous iteration (or by
-        ``adopt_state`` / ``reset``), and leaves A x of the new iterate there."""
+        ``adopt_state`` / ``reset``), and leaves A x of the new iterate there;
+        A^T y comes from ``aty``, so ``f.y`` must not change between
+        ``adopt_state`` / ``reset`` and the iterations that use it."""
         if not self.epi:
             self._enqueue_nd()
             return
         A, plan, s = self.A, self.A.plan, dev.stream()
         P = lambda t: t.data_ptr()
         y = self.admm.f.y
-        # x-step: rhs = A^T y + rho C^T (z - u)
-        call("scxr_xr2d_adjoint_epi", plan, P(y), P(self.rhs), 2, self.rho, self.circ, None,
-             P(self.z), P(self.u), None, None, s)
+        # x-step: rhs = A^T y + rho C^T (z - u), A^T y kept from the start
+        # (the same operations as the adjoint's right-hand-side epilogue)
+        call("scxr_admm_rhs_nd", 2, self.shape_c, P(self.aty), P(self.z), P(self.u), self.rho,
+             self.circ, P(self.rhs), s)
         # A_lhs x0 = rho C^T C x + A^T A x, with A x from the last iteration
         call("scxr_xr2d_adjoint_epi", plan, P(self.ax), P(self.Ap), 1, self.rho, self.circ,
              P(self.x), None, None, None, None, s)
@@ -371,8 +382,7 @@ class _FusedTV:
         A, s, nd, sh = self.A, dev.stream(), self.ndim, self.shape_c
         P = lambda t: t.data_ptr()
         y = self.admm.f.y
-        self._bwd(y, out=self.rhs)
-        call("scxr_admm_rhs_nd", nd, sh, P(self.rhs), P(self.z), P(self.u), self.rho, self.circ,
+        call("scxr_admm_rhs_nd", nd, sh, P(self.aty), P(self.z), P(self.u), self.rho, self.circ,
              P(self.rhs), s)
         self._bwd(self.ax, out=self.Ap)  # A x from the last iteration
         call("scxr_admm_lhs_nd", nd, sh, P(self.x), P(self.Ap), self.rho, self.circ, P(self.Ap),
