@@ -264,6 +264,11 @@ int32_t scxr_elem_blocks(int64_t n);
 int scxr_admm_zu2d(int32_t M, int32_t N, const float* x, float* z, float* u, float* zold,
                    float thr, int32_t circular, int32_t isotropic, double* ws_prml,
                    double* ws_g, void* stream);
+/* x-step right-hand side of ADMM-TV for an M x N image from a kept
+ * aty = A^T y: out = aty + rho C^T (z - u) (_admmaux.py:245-256), the same
+ * values as scxr_xr2d_adjoint_epi kind 2 on y without back projecting y. */
+int scxr_admm_rhs2d(int32_t M, int32_t N, const float* aty, const float* z, const float* u,
+                    float rho, int32_t circular, float* out, void* stream);
 /* Partials of ||rho * C^T (z - zold)||^2 (dual residual, _admm.py:279-295). */
 int scxr_admm_dual2d(int32_t M, int32_t N, const float* z, const float* zold, float rho,
                      int32_t circular, double* ws, void* stream);

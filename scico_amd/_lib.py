@@ -123,6 +123,8 @@ SIGNATURES = [
       c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
     ("scxr_elem_blocks", c_int32, [c_int64]),
     # ADMM
+    ("scxr_admm_rhs2d", c_int32,
+     [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
     ("scxr_admm_zu2d", c_int32,
      [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32,
       c_void_p, c_void_p, c_void_p]),

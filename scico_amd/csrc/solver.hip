@@ -480,6 +480,28 @@ __device__ __forceinline__ float adj_nd(const float* __restrict__ a, const float
   return acc;
 }
 
+// out = aty + rho C^T (z - u) for an M x N image: the operations of the 2D
+// adjoint's right-hand-side epilogue (_admmaux.py:245-256), from a kept
+// A^T y (workgroup vb of vg).
+__device__ __forceinline__ void admm_rhs2d_body(int M, int N, const float* __restrict__ aty,
+                                                const float* __restrict__ z,
+                                                const float* __restrict__ u, float rho, int circ,
+                                                float* __restrict__ out, int vb, int vg) {
+  const int64_t n = static_cast<int64_t>(M) * N;
+  for (int64_t e = vb * 256ll + threadIdx.x; e < n; e += 256ll * vg) {
+    const int i = static_cast<int>(e / N), j = static_cast<int>(e - static_cast<int64_t>(i) * N);
+    out[e] = __fadd_rn(aty[e], __fmul_rn(rho, fd::adj2d_diff(z, u, M, N, i, j, circ != 0)));
+  }
+}
+
+__global__ __launch_bounds__(256) void admm_rhs2d_kernel(int M, int N,
+                                                         const float* __restrict__ aty,
+                                                         const float* __restrict__ z,
+                                                         const float* __restrict__ u, float rho,
+                                                         int circ, float* __restrict__ out) {
+  admm_rhs2d_body(M, N, aty, z, u, rho, circ, out, blockIdx.x, gridDim.x);
+}
+
 // out = base + rho C^T (z - u)   (_admmaux.py:245-256); out may alias base.
 __global__ __launch_bounds__(256) void admm_rhs_nd_kernel(int ndim, int d0, int d1, int d2,
                                                           const float* base,
@@ -1153,20 +1175,13 @@ __global__ __launch_bounds__(256) void admm_dual_b(SegTabs t, const float* __res
                    q.vg);
 }
 
-// out = aty + rho C^T (z - u) per slice: the operations of the 2D adjoint's
-// right-hand-side epilogue (_admmaux.py:245-256), from a kept A^T y.
 __global__ __launch_bounds__(256) void admm_rhs_b(SegTabs t, const float* __restrict__ aty,
                                                   const float* __restrict__ z,
                                                   const float* __restrict__ u, float rho,
                                                   int circ, float* __restrict__ out) {
   const SegBlock q = seg_block(t);
-  const float* zs = z + 2 * q.o;
-  const float* us = u + 2 * q.o;
-  for (int64_t e = q.vb * 256ll + threadIdx.x; e < q.n; e += 256ll * q.vg) {
-    const int i = static_cast<int>(e / q.N), j = static_cast<int>(e - static_cast<int64_t>(i) * q.N);
-    out[q.o + e] = __fadd_rn(aty[q.o + e],
-                             __fmul_rn(rho, fd::adj2d_diff(zs, us, q.M, q.N, i, j, circ != 0)));
-  }
+  admm_rhs2d_body(q.M, q.N, aty + q.o, z + 2 * q.o, u + 2 * q.o, rho, circ, out + q.o, q.vb,
+                  q.vg);
 }
 
 __global__ __launch_bounds__(256) void admm_itstat_b(
@@ -1355,6 +1370,16 @@ int scxr_cg_iter_t(int32_t M, int32_t N, int32_t k, int32_t maxiter, float* x, f
                      as_stream(stream), M, N, k, maxiter, r, p, pT, state, flags, ws,
                      static_cast<int>(g));
   SCXR_LAUNCHED("cg p (with transposed copy)");
+  return SCXR_OK;
+}
+
+int scxr_admm_rhs2d(int32_t M, int32_t N, const float* aty, const float* z, const float* u,
+                    float rho, int32_t circular, float* out, void* stream) {
+  SCXR_REQUIRE(M > 0 && N > 0 && aty && z && u && out, SCXR_ERR_ARG, "bad admm_rhs2d arguments");
+  const int64_t n = static_cast<int64_t>(M) * N;
+  hipLaunchKernelGGL(solver::admm_rhs2d_kernel, dim3(solver::elem_grid(n)), dim3(256), 0,
+                     as_stream(stream), M, N, aty, z, u, rho, circular, out);
+  SCXR_LAUNCHED("admm rhs2d");
   return SCXR_OK;
 }
 

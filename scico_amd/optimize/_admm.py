@@ -341,7 +341,7 @@ class _FusedTV:
         y = self.admm.f.y
         # x-step: rhs = A^T y + rho C^T (z - u), A^T y kept from the start
         # (the same operations as the adjoint's right-hand-side epilogue)
-        call("scxr_admm_rhs_nd", 2, self.shape_c, P(self.aty), P(self.z), P(self.u), self.rho,
+        call("scxr_admm_rhs2d", self.M, self.N, P(self.aty), P(self.z), P(self.u), self.rho,
              self.circ, P(self.rhs), s)
         # A_lhs x0 = rho C^T C x + A^T A x, with A x from the last iteration
         call("scxr_xr2d_adjoint_epi", plan, P(self.ax), P(self.Ap), 1, self.rho, self.circ,

@@ -191,3 +191,21 @@ def test_cg_iter_t_and_forward_t(gpu):
     _lib.call("scxr_xr2d_forward_t", H.plan, im.data_ptr(), imT.data_ptr(), s2.data_ptr(), None,
               None)
     assert torch.equal(s1, s2)
+
+
+@pytest.mark.parametrize("circ", [0, 1])
+def test_rhs2d_equals_rhs_epilogue(setup, circ):
+    """A^T y kept and rho C^T (z - u) added in the elementwise pass equals
+    the adjoint's right-hand-side epilogue bit for bit."""
+    d = setup
+    H = d["H"]
+    y, z, u = dev(d["y"]), dev(d["z"]), dev(d["u"])
+    epi = torch.empty((N, N), device="cuda")
+    _lib.call("scxr_xr2d_adjoint_epi", H.plan, y.data_ptr(), epi.data_ptr(), 2, RHO, circ, None,
+              z.data_ptr(), u.data_ptr(), None, None, None)
+    aty = torch.empty((N, N), device="cuda")
+    _lib.call("scxr_xr2d_adjoint", H.plan, y.data_ptr(), aty.data_ptr(), None, None)
+    out = torch.full((N, N), -3.0, device="cuda")
+    _lib.call("scxr_admm_rhs2d", N, N, aty.data_ptr(), z.data_ptr(), u.data_ptr(), RHO, circ,
+              out.data_ptr(), None)
+    assert torch.equal(out, epi)
