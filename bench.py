@@ -7,7 +7,9 @@ SquaredL2Loss + 2e-2 * L21(FiniteDifference(append=0)), rho = 1 and a
 10-step CG x-step, starting from x0 = 0.
 
 value: iterations/s with fixed work per iteration (CG tol 0, so exactly 10
-CG steps = 12 forward + 12 back projections, plus the iteration statistics),
+CG steps; 11 forward + 11 back projections, since an iteration keeps A x of
+its residual projection and A^T y instead of recomputing them, plus the
+iteration statistics),
 timed over --steps iterations replayed from one captured HIP graph.  The
 configured-tolerance run (CG tol 1e-4, 100 iterations, where warm-started
 CG stops early) is reported beside it with its mean CG iteration count.
@@ -513,7 +515,7 @@ def main():
     ang, xgt, y, H = build_problem(N, A, D, 20, 0)
     use_graph = not args.no_graph
     # Headline: fixed work per iteration (CG tol 0: exactly 10 CG steps, i.e.
-    # 12 forward + 12 back projections per ADMM iteration), from x0 = 0.
+    # 11 forward + 11 back projections per ADMM iteration), from x0 = 0.
     solver = make_solver(H, y, N, lam, rho, cg_maxiter, args.steps + args.warmup, cg_tol=0.0)
     log(f"bench: timing {args.steps} fixed-work ADMM iterations ({'graph' if use_graph else 'eager'})")
     dt, hist = run_timed(solver, args.steps, args.warmup, use_graph)
