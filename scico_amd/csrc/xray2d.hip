@@ -704,6 +704,14 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
   set_t(s0);
   // Accumulate block (s0, c0) from cur after issuing the loads of the next
   // block into nxt; false once the last block is done.
+  // Lane l's pixel of phase r in block c0 is c0 + loff + r.  c0 and loff + r
+  // are below 2^24 (plan creation checks the image size), so both convert
+  // exactly and float(c0) + float(loff + r) rounds the same integer once, as
+  // the conversion of the sum does: one add per phase instead of an integer
+  // add and a conversion.
+  float flr[S];
+#pragma unroll
+  for (int r = 0; r < S; ++r) flr[r] = static_cast<float>(loff + r);
   auto step = [&](const float (&cur)[CH][S], float (&nxt)[CH][S]) {
     int sn = s0;
     unsigned cn = c0 + span;
@@ -716,15 +724,19 @@ __device__ __forceinline__ void fwd_body_s(const float* __restrict__ src, float*
       if (sn != s0) set_lines(sn);
       load(nxt, cn);
     }
+    const float fc0 = static_cast<float>(c0);
 #pragma unroll
     for (int r = 0; r < S; ++r) {
       float val[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) val[c] = cur[c][r];
-      fwd_phase<V, CH, COLFAST, FR>(acc, row, t, val, static_cast<float>(c0 + loff + r), cnt, Pxmin,
-                                P0, P1, rw, Dmax);
+      fwd_phase<V, CH, COLFAST, FR>(acc, row, t, val, __fadd_rn(fc0, flr[r]), cnt, Pxmin, P0, P1,
+                                    rw, Dmax);
     }
-    if (more && sn != s0) set_t(sn);
+    // a new line every few blocks: branch (hinted) rather than recompute the
+    // line terms every block and select, which is what the compiler does
+    // otherwise
+    if (__builtin_expect(more && sn != s0, 0)) set_t(sn);
     s0 = sn;
     c0 = cn;
     return more;
