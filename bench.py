@@ -51,12 +51,12 @@ VALU_PEAK = 256 * 64 * 2.4e9
 # by the counters of a 64 MiB copy (FETCH_SIZE reads 0.5x the bytes moved,
 # WRITE_SIZE 1.0x): forward = image transpose + projection.
 TRAFFIC_BYTES = {
-    "forward": 1024 * ((524.0 + 8761.1) * 2.0 + (1024.0 + 2047.3) * 1.0),
-    "adjoint": 1024 * (4929.2 * 2.0 + 1024.0 * 1.0),
+    "forward": 1024 * ((524.1 + 9084.7) * 2.0 + (1024.0 + 2047.3) * 1.0),
+    "adjoint": 1024 * (4936.2 * 2.0 + 1024.0 * 1.0),
 }
 # VALU instructions (lane-instructions) per (pixel, view) interaction,
 # rocprofv3 --pmc SQ_INSTS_VALU x 64 / interactions, same workload.
-VALU_PER_INTERACTION = {"forward": 14.21, "adjoint": 13.25}
+VALU_PER_INTERACTION = {"forward": 13.27, "adjoint": 13.25}
 # the profile these counter readings come from (commands and raw figures)
 COUNTERS_FROM = "profiles/xr2d_projector_512x512_720views_det724.md"
 
