@@ -312,9 +312,10 @@ class ADMMTVBatch:
             raise RuntimeError("the batch plan was re-created (autotune) after this solver was "
                                "built; build a new ADMMTVBatch")
         F = self.maxiter + 1
-        if self._aty_key != H.config:
+        key = (H.config, self.y._version)
+        if self._aty_key != key:  # new configuration, or y modified in place
             H.back_project(self.y, out=self.aty)
-            self._aty_key = H.config
+            self._aty_key = key
         call("scxr_batch_admm_rhs", H.plan, P(self.aty), P(self.z), P(self.u), self.rho,
              self.circ, P(self.rhs), s)
         H.back_project(self.ax, out=self.Ap, kind=1, rho=self.rho, circular=self.circ, p=self.x)

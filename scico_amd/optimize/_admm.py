@@ -272,15 +272,38 @@ class _FusedTV:
 
     def _project_x(self):
         """ax = A x for the iterate just set and aty = A^T y (device work only)."""
-        y = self.admm.f.y
         if self.epi:
             call("scxr_xr2d_forward", self.A.plan, self.x.data_ptr(), self.ax.data_ptr(), None,
                  dev.stream())
+        else:
+            self._fwd(self.x, out=self.ax)
+        self._ax_ver = self.x._version  # the kernels' own writes do not count
+        self._back_project_y()
+
+    def _back_project_y(self):
+        """aty = A^T y, noting which y (and which version of it) it came from."""
+        y = self.admm.f.y
+        if self.epi:
             call("scxr_xr2d_adjoint", self.A.plan, y.data_ptr(), self.aty.data_ptr(), None,
                  dev.stream())
         else:
-            self._fwd(self.x, out=self.ax)
             self._bwd(y, out=self.aty)
+        self._aty_of = (y, y._version)
+
+    def _check_inputs(self):
+        """Host-side checks before an iteration is enqueued (a captured
+        iteration replays what it was captured with): a new iterate assigned
+        to ``admm.x`` is adopted, one modified in place by torch operations
+        is projected again, and y is back projected again if ``f.y`` was
+        replaced or modified in place since aty was computed."""
+        if self.admm.x is not self.x:
+            self.adopt_state()
+        elif self.x._version != self._ax_ver:
+            self._project_x()
+        y = self.admm.f.y
+        src, ver = self._aty_of
+        if y is not src or y._version != ver:
+            self._back_project_y()
 
     def _publish(self):
         a = self.admm
@@ -331,8 +354,10 @@ class _FusedTV:
         """Enqueue one full ADMM iteration (no host synchronisation; capturable).
         It starts from ``ax`` = A x, left by the previous iteration (or by
         ``adopt_state`` / ``reset``), and leaves A x of the new iterate there;
-        A^T y comes from ``aty``, so ``f.y`` must not change between
-        ``adopt_state`` / ``reset`` and the iterations that use it."""
+        A^T y comes from ``aty``, back projected again when ``f.y`` has been
+        replaced or modified in place since (checked on the host here, so a
+        captured graph keeps the A^T y it was captured with)."""
+        self._check_inputs()
         if not self.epi:
             self._enqueue_nd()
             return
