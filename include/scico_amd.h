@@ -119,7 +119,9 @@ int scxr_ramp_filter(int32_t A, int32_t D, int32_t Dout, int32_t shift, const fl
  * equal to the batched projector's for the same slice.  The effective
  * count (always 1 for the general-geometry forward) is written to
  * *effective when it is not null.  Calls on one plan on different streams
- * share its workspace and must not overlap. */
+ * share its workspace and must not overlap.  Set the split before capturing
+ * a graph that runs the plan's forward: a captured launch keeps the grid and
+ * workspace of its capture, and a larger split replaces the workspace. */
 int scxr_xr2d_set_forward_split(scxr_xr2d_plan* plan, int32_t parts, int32_t* effective);
 
 /* Launch facts: out[0] forward workgroups (= partial count of

@@ -133,7 +133,9 @@ class XRayTransform2D(LinearOperator):
         (1..8; 0 = the automatic choice) and return the count in effect.
         The parts' partial sinograms are added in a fixed order; a split
         changes float32 rounding only.  With 1 the forward is bitwise equal
-        to the batched projector's for the same slice."""
+        to the batched projector's for the same slice.  Call it before
+        capturing a graph that projects with this operator (a captured
+        launch keeps its grid, and a larger split replaces the workspace)."""
         eff = ctypes.c_int32(0)
         call("scxr_xr2d_set_forward_split", self.plan, int(parts), ctypes.byref(eff))
         return int(eff.value)
