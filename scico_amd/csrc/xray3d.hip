@@ -430,14 +430,15 @@ __global__ __launch_bounds__(256) void adj_slice_staged_kernel(
   stage_view(0, rl(lo_c, 0), rl(n_c, 0), 0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  int ul = 0;
+  int cab = 0;  // the pixel's two detector rows, clamped as adj_slice_kernel
+                // clamps them, packed as two 16-bit fields (U < 65536)
   float wl = 0.0f, wr = 0.0f;
 #pragma unroll 1
   for (int v = 0; v < V; ++v) {
     const int j = v & 7;
     if (j == 0) {  // taps of the wave's 8 pixels for views v .. v + 7
       const int vv = v + jv;
-      ul = 0;
+      int ul = 0;
       wl = wr = 0.0f;
       if (vv < V) {
         const float4 g = geom[vv];
@@ -446,6 +447,7 @@ __global__ __launch_bounds__(256) void adj_slice_staged_kernel(
         wl = pok ? tp.wl : 0.0f;
         wr = pok ? tp.wr : 0.0f;
       }
+      cab = min(max(ul, 0), U - 1) | (min(max(ul + 1, 0), U - 1) << 16);
     }
     // this view's range, then the next view's, from the 16-view range sets
     const int lo = rl(lo_c, 4 * (v & 15));
@@ -458,13 +460,18 @@ __global__ __launch_bounds__(256) void adj_slice_staged_kernel(
     }
     // both rows of all eight pixels first (no reuse branches: LDS reads
     // are cheap), then the accumulation in adj_slice_kernel's order
-    const float4* sb = &stage[v & 1][0][0] + lane;
+    const char* sb = reinterpret_cast<const char*>(&stage[v & 1][0][0] + lane);
+    // byte offsets of both staged rows (64 float4 = 1 KiB apart): every
+    // pixel's clamped rows lie in [lo, lo + n), n <= ADJ_NW (the range is
+    // taken over the tile's corner bins), so neither 16-bit field borrows
+    // and each offset stays below 2^14
+    const unsigned rab = static_cast<unsigned>(cab - (lo | (lo << 16))) << 10;
     float4 ya[AR], yb[AR];
 #pragma unroll
     for (int i = 0; i < AR; ++i) {
-      const int u = rl(ul, j * AR + i);
-      ya[i] = sb[(min(max(u, 0), U - 1) - lo) * 64];
-      yb[i] = sb[(min(max(u + 1, 0), U - 1) - lo) * 64];
+      const unsigned x = static_cast<unsigned>(rl(static_cast<int>(rab), j * AR + i));
+      ya[i] = *reinterpret_cast<const float4*>(sb + (x & 0xffffu));
+      yb[i] = *reinterpret_cast<const float4*>(sb + (x >> 16));
     }
 #pragma unroll
     for (int i = 0; i < AR; ++i) {
@@ -887,8 +894,9 @@ int create(Plan& p, int X, int Y, int Z, int V, const float* mats, int D0, int D
     // Staged adjoint: a 4 x 8 tile's rows per view, at most
     // ceil(3 |a| + 7 |b| + rounding) + 2 (the corner bins' spread plus the
     // right-hand tap), must fit ADJ_NW.  The rounding margin covers the
-    // float32 evaluation of the corner centres (a few ulps of |P|).
-    p.adj_staged = !(allow_slice & 2);
+    // float32 evaluation of the corner centres (a few ulps of |P|).  The
+    // kernel packs two detector-row indices into 16-bit fields: U < 65536.
+    p.adj_staged = !(allow_slice & 2) && p.U < 65536;
     for (int v = 0; v < V && p.adj_staged; ++v) {
       const double a = std::fabs(static_cast<double>(g[v].x)),
                    b = std::fabs(static_cast<double>(g[v].y));
