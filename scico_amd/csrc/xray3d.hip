@@ -251,6 +251,9 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
       // issue no load (an out-of-range load would still take its cycles
       // in the texture addresser, which bounds this kernel).
       constexpr int NG = FB + 1, GC = 5;
+      // byte offset of slot L: (rpix + (L - base) kstep) rowb, formed once
+      // per group and stepped by kstep rowb from slot to slot
+      const int rbyte = rpix * rowb, kbyte = kstep * rowb;
 #pragma unroll
       for (int g0 = 0; g0 < NG; g0 += GC) {
         float4 xs[GC][SPG];
@@ -258,11 +261,12 @@ __global__ __launch_bounds__(256) void fwd_slice_kernel(const float4* __restrict
         for (int c = 0; c < GC; ++c) {
           const int gg = g0 + c - 1;
           if (gg < FB) {
+            const int gbyte = rbyte + (first[gg + 1] - base) * kbyte;
 #pragma unroll
             for (int k = 0; k < SPG; ++k) {
               const int L = first[gg + 1] + k;
               // an empty slot's register is never read
-              if (L < first[gg + 2]) xs[c][k] = bload4(vr, qoff, (rpix + (L - base) * kstep) * rowb);
+              if (L < first[gg + 2]) xs[c][k] = bload4(vr, qoff, gbyte + k * kbyte);
             }
           }
         }
